@@ -1,0 +1,250 @@
+/*
+ * magma_mi355x.h -- C ABI of the MI355X-native batched LU factor/solve library
+ * (libmagma_mi355x.so).
+ *
+ * Every declaration below keeps the exact signature, argument meaning, return
+ * convention and enum values of the MAGMA 2.10 routine it replaces, so that a
+ * caller compiled against MAGMA's headers links against this library instead.
+ * Each entry cites the MAGMA prototype (file:line, relative to the MAGMA source
+ * tree) whose double-precision ("d") codegen form it replaces.
+ *
+ * magma_int_t is 32-bit (MAGMA's default LP64 build, include/magma_types.h:68-72).
+ * Device pointers are plain HIP device addresses; pointer arrays (double **,
+ * magma_int_t **) live in device memory, as in MAGMA.
+ */
+#ifndef MAGMA_MI355X_H
+#define MAGMA_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Types (include/magma_types.h:68-72, 100, 227)                              */
+/* ------------------------------------------------------------------------- */
+typedef int            magma_int_t;
+typedef magma_int_t    magma_device_t;
+typedef double         real_Double_t;
+typedef void*          magma_ptr;
+typedef double*        magmaDouble_ptr;
+typedef float*         magmaFloat_ptr;
+typedef magma_int_t*   magmaInt_ptr;
+
+struct magma_queue;                       /* opaque: wraps one hipStream_t */
+typedef struct magma_queue* magma_queue_t;
+
+/* Return codes (include/magma_types.h:549-568) */
+#define MAGMA_SUCCESS               0
+#define MAGMA_ERR                  -100
+#define MAGMA_ERR_NOT_INITIALIZED  -101
+#define MAGMA_ERR_NOT_SUPPORTED    -103
+#define MAGMA_ERR_HOST_ALLOC       -112
+#define MAGMA_ERR_DEVICE_ALLOC     -113
+#define MAGMA_ERR_INVALID_PTR      -115
+#define MAGMA_ERR_UNKNOWN          -116
+#define MAGMA_ERR_NOT_IMPLEMENTED  -117
+
+/* Parameter enums (include/magma_types.h:613-640), same numeric values. */
+typedef enum { MagmaFalse = 0, MagmaTrue = 1 } magma_bool_t;
+typedef enum { MagmaNoTrans = 111, MagmaTrans = 112, MagmaConjTrans = 113 } magma_trans_t;
+typedef enum { MagmaUpper = 121, MagmaLower = 122, MagmaFull = 123 } magma_uplo_t;
+typedef enum { MagmaNonUnit = 131, MagmaUnit = 132 } magma_diag_t;
+typedef enum { MagmaLeft = 141, MagmaRight = 142 } magma_side_t;
+
+/* Mixed-precision refinement selectors (include/magma_types.h:861-898).
+ * The reference values are kept; the two BF16 factorisation kinds are new
+ * (MAGMA has no BF16 LU) and use values the reference leaves unassigned. */
+typedef enum {
+    Magma_PREC_SS         = 900,   /* FP32 LU, FP32 GEMM                          */
+    Magma_PREC_SHT        = 905,   /* FP32 LU, FP16-input GEMM, FP32 accumulate    */
+    Magma_PREC_XSH_S      = 915,
+    Magma_PREC_XSH_163232TC = 917,
+    Magma_PREC_SBT        = 940,   /* new: FP32 LU, BF16-input GEMM, FP32 accum.   */
+    Magma_PREC_SB3T       = 941,   /* new: FP32 LU, 3-term BF16 split GEMM         */
+    Magma_REFINE_IRSTRS   = 920,   /* classical IR, low-precision triangular solve */
+    Magma_REFINE_IRDTRS   = 921,
+    Magma_REFINE_IRGMSTRS = 922,
+    Magma_REFINE_IRGMDTRS = 923,
+    Magma_REFINE_GMSTRS   = 924,   /* FGMRES preconditioned by the low-prec. LU    */
+    Magma_REFINE_GMDTRS   = 925
+} magma_refinement_t;
+
+/* ------------------------------------------------------------------------- */
+/* Runtime (include/magma_auxiliary.h, include/magma_copy.h)                  */
+/* ------------------------------------------------------------------------- */
+/* include/magma_auxiliary.h:24-25 */
+magma_int_t magma_init(void);
+magma_int_t magma_finalize(void);
+
+/* include/magma_auxiliary.h:48-49 */
+real_Double_t magma_wtime(void);
+real_Double_t magma_sync_wtime(magma_queue_t queue);
+
+/* include/magma_auxiliary.h:82-103 (magma_free is a macro over _internal) */
+magma_int_t magma_malloc(magma_ptr *ptr_ptr, size_t bytes);
+magma_int_t magma_malloc_cpu(void **ptr_ptr, size_t bytes);
+magma_int_t magma_free_cpu(void *ptr);
+magma_int_t magma_free_internal(magma_ptr ptr, const char *func, const char *file, int line);
+#define magma_free(ptr) magma_free_internal(ptr, __func__, __FILE__, __LINE__)
+
+/* include/magma_auxiliary.h:230 */
+void magma_getdevice(magma_device_t *dev);
+void magma_setdevice(magma_device_t dev);
+
+/* include/magma_auxiliary.h:259-300 (the public names are macros) */
+void magma_queue_create_internal(magma_device_t device, magma_queue_t *queue_ptr,
+                                 const char *func, const char *file, int line);
+/* MAGMA's HIP form also takes hipBLAS/hipSPARSE handles; this library uses no
+ * vendor BLAS, so those two arguments are accepted and ignored. */
+void magma_queue_create_from_hip_internal(magma_device_t device, void *hip_stream,
+                                          void *hipblas_handle, void *hipsparse_handle,
+                                          magma_queue_t *queue_ptr,
+                                          const char *func, const char *file, int line);
+void magma_queue_destroy_internal(magma_queue_t queue, const char *func, const char *file, int line);
+void magma_queue_sync_internal(magma_queue_t queue, const char *func, const char *file, int line);
+/* include/magma_types.h:227 */
+void *magma_queue_get_hip_stream(magma_queue_t queue);
+magma_device_t magma_queue_get_device(magma_queue_t queue);
+#define magma_queue_create(dev, qp)  magma_queue_create_internal(dev, qp, __func__, __FILE__, __LINE__)
+#define magma_queue_destroy(q)       magma_queue_destroy_internal(q, __func__, __FILE__, __LINE__)
+#define magma_queue_sync(q)          magma_queue_sync_internal(q, __func__, __FILE__, __LINE__)
+
+/* include/magma_copy.h:26-30 and the matrix forms */
+void magma_setvector_internal(magma_int_t n, magma_int_t elemSize, const void *hx_src, magma_int_t incx,
+                              magma_ptr dy_dst, magma_int_t incy, magma_queue_t queue,
+                              const char *func, const char *file, int line);
+void magma_getvector_internal(magma_int_t n, magma_int_t elemSize, const void *dx_src, magma_int_t incx,
+                              void *hy_dst, magma_int_t incy, magma_queue_t queue,
+                              const char *func, const char *file, int line);
+void magma_setmatrix_internal(magma_int_t m, magma_int_t n, magma_int_t elemSize,
+                              const void *hA_src, magma_int_t lda, magma_ptr dB_dst, magma_int_t lddb,
+                              magma_queue_t queue, const char *func, const char *file, int line);
+void magma_getmatrix_internal(magma_int_t m, magma_int_t n, magma_int_t elemSize,
+                              const void *dA_src, magma_int_t ldda, void *hB_dst, magma_int_t ldb,
+                              magma_queue_t queue, const char *func, const char *file, int line);
+
+/* control/xerbla.cpp:51, include/magma_auxiliary.h:357-359 */
+void magma_xerbla(const char *srname, magma_int_t neg_info);
+const char *magma_strerror(magma_int_t error);
+
+/* include/magma_zbatched.h:28 and include/magma_batched.h:63: build a device
+ * pointer array out[i] = input + i*batch_offset + row + column*lda. */
+void magma_dset_pointer(double **output_array, double *input, magma_int_t lda,
+                        magma_int_t row, magma_int_t column, magma_int_t batch_offset,
+                        magma_int_t batchCount, magma_queue_t queue);
+void magma_iset_pointer(magma_int_t **output_array, magma_int_t *input, magma_int_t lda,
+                        magma_int_t row, magma_int_t column, magma_int_t batchSize,
+                        magma_int_t batchCount, magma_queue_t queue);
+
+/* Block-size queries (control/get_batched_crossover.cpp:300,336). The values
+ * describe this library's kernels, not MAGMA's. */
+void magma_get_dgetrf_batched_nbparam(magma_int_t n, magma_int_t *nb, magma_int_t *recnb);
+void magma_get_dgetrf_vbatched_nbparam(magma_int_t max_m, magma_int_t max_n, magma_int_t *nb, magma_int_t *recnb);
+
+/* ------------------------------------------------------------------------- */
+/* Batched LU (the hot path)                                                  */
+/* ------------------------------------------------------------------------- */
+/* include/magma_zbatched.h:848 -- P*A = L*U for batchCount m-by-n matrices.
+ * ipiv_array[i] holds min(m,n) 1-based LAPACK interchanges; info_array[i] is
+ * 0 or the 1-based index of the first exactly-zero pivot. Host-synchronous at
+ * return, as in MAGMA. */
+magma_int_t magma_dgetrf_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t ldda,
+                                 magma_int_t **ipiv_array, magma_int_t *info_array,
+                                 magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:464 -- solve op(A) X = B with the factors of
+ * magma_dgetrf_batched. trans = MagmaTrans follows LAPACK dgetrs('T')
+ * (src/zgetrs_gpu.cpp:203-227), not the batched reference's transposed branch.
+ * Asynchronous on queue. */
+magma_int_t magma_dgetrs_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                                 double **dA_array, magma_int_t ldda, magma_int_t **dipiv_array,
+                                 double **dB_array, magma_int_t lddb,
+                                 magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zbatched.h:1009 -- A X = B; A is overwritten by its LU factors,
+ * B by X. Asynchronous on queue. */
+magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                magma_int_t **dipiv_array, double **dB_array, magma_int_t lddb,
+                                magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zvbatched.h:57 -- variable-size batched LU. m, n, ldda are
+ * device arrays of batchCount entries. Host-synchronous at return. */
+magma_int_t magma_dgetrf_vbatched(magma_int_t *m, magma_int_t *n, double **dA_array, magma_int_t *ldda,
+                                  magma_int_t **ipiv_array, magma_int_t *info_array,
+                                  magma_int_t batchCount, magma_queue_t queue);
+
+/* include/magma_zvbatched.h:73 -- asynchronous form with caller workspace;
+ * *lwork < 0 is a workspace query (bytes). The max_* arguments bound the
+ * batch (max_m >= every m[i], and so on). */
+magma_int_t magma_dgetrf_vbatched_max_nocheck_work(magma_int_t *m, magma_int_t *n,
+        magma_int_t max_m, magma_int_t max_n, magma_int_t max_minmn, magma_int_t max_mxn,
+        double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array, magma_int_t *info_array,
+        void *work, magma_int_t *lwork, magma_int_t batchCount, magma_queue_t queue);
+
+/* No MAGMA counterpart (MAGMA has no getrs/gesv for variable sizes). Solve
+ * A_i X_i = B_i with the factors of magma_dgetrf_vbatched; n, nrhs, ldda, lddb
+ * are device arrays; max_n / max_nrhs bound them. Asynchronous. */
+magma_int_t magma_dgetrs_vbatched(magma_trans_t trans, magma_int_t *n, magma_int_t *nrhs,
+        magma_int_t max_n, magma_int_t max_nrhs,
+        double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+        double **dB_array, magma_int_t *lddb, magma_int_t batchCount, magma_queue_t queue);
+
+/* No MAGMA counterpart: factor and solve variable-size systems; A_i is
+ * overwritten by its LU factors and B_i by X_i. Asynchronous. */
+magma_int_t magma_dgesv_vbatched(magma_int_t *n, magma_int_t *nrhs,
+        magma_int_t max_n, magma_int_t max_nrhs,
+        double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+        double **dB_array, magma_int_t *lddb, magma_int_t *dinfo_array,
+        magma_int_t batchCount, magma_queue_t queue);
+
+/* ------------------------------------------------------------------------- */
+/* Mixed-precision iterative refinement                                        */
+/* ------------------------------------------------------------------------- */
+/* include/magma_htc.h:27 -- FP16-input/FP32-accumulate LU, then FGMRES in
+ * FP64 preconditioned by it (src/dxgesv_gmres_gpu.cpp:1122-1157). ipiv is host
+ * memory, dipiv device memory; dworkd >= n*nrhs doubles, dworks >= n*(n+nrhs)
+ * floats. iter > 0: refinement converged in iter steps; iter < 0: FP64
+ * fallback was used (-2 overflow on down-cast, -3 low-precision LU failed,
+ * -k no convergence after k steps). Host-synchronous. */
+magma_int_t magma_dhgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+        magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv, magmaInt_ptr dipiv,
+        magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
+        magmaDouble_ptr dworkd, magmaFloat_ptr dworks, magma_int_t *iter, magma_int_t *info);
+
+/* src/dxgesv_gmres_gpu.cpp:160 -- expert driver: facto_type in {SS, SHT,
+ * XSH_163232TC (same as SHT), SBT, SB3T}, solver_type in {IRSTRS, GMSTRS}.
+ * facto_time (seconds) may be NULL. */
+magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+        magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv, magmaInt_ptr dipiv,
+        magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
+        magmaDouble_ptr dworkd, magmaFloat_ptr dworks,
+        magma_refinement_t facto_type, magma_refinement_t solver_type,
+        magma_int_t *iter, magma_int_t *info, real_Double_t *facto_time);
+
+/* No MAGMA counterpart: BF16-input/FP32-accumulate LU + FP64 refinement with
+ * the same contract as magma_dhgesv_iteref_gpu. */
+magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+        magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv, magmaInt_ptr dipiv,
+        magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
+        magmaDouble_ptr dworkd, magmaFloat_ptr dworks, magma_int_t *iter, magma_int_t *info);
+
+/* No MAGMA counterpart: batched mixed-precision solve of A_i X_i = B_i
+ * (n <= 256, any nrhs), one fused launch per batch. A_i and B_i are read only;
+ * X_i is written. facto_type selects the low-precision LU (SHT, SBT, SB3T, SS),
+ * solver_type IRSTRS or GMSTRS; itermax bounds refinement steps.
+ * iter_array[i] follows the iter convention of magma_dhgesv_iteref_gpu;
+ * a system that does not converge is re-solved in FP64 inside the launch.
+ * info_array[i] > 0 if the FP64 fallback found an exactly-zero pivot. */
+magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs,
+        double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+        double **dX_array, magma_int_t lddx,
+        magma_refinement_t facto_type, magma_refinement_t solver_type, magma_int_t itermax,
+        magma_int_t *iter_array, magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAGMA_MI355X_H */

@@ -1,0 +1,78 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+the public header declares, and rejects bad arguments with MAGMA's codes
+before touching the device."""
+import ctypes
+
+import pytest
+
+import magma_amd._ffi as ffi
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return ffi.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = ffi.header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_abi_enum_values():
+    text = ffi.HEADER.read_text()
+    for token in ("MagmaNoTrans = 111", "MagmaTrans = 112", "MagmaConjTrans = 113",
+                  "Magma_PREC_SHT        = 905", "Magma_REFINE_GMSTRS   = 924",
+                  "Magma_REFINE_IRSTRS   = 920", "typedef int            magma_int_t"):
+        assert token in text
+
+
+def test_getrf_batched_argument_codes(lib):
+    # src/zgetrf_batched.cpp:94-105: m<0 -> -1, n<0 -> -2, ldda<max(1,m) -> -4
+    assert lib.magma_dgetrf_batched(-1, 4, None, 4, None, None, 1, None) == -1
+    assert lib.magma_dgetrf_batched(4, -1, None, 4, None, None, 1, None) == -2
+    assert lib.magma_dgetrf_batched(4, 4, None, 3, None, None, 1, None) == -4
+    # quick return without touching memory
+    assert lib.magma_dgetrf_batched(0, 4, None, 1, None, None, 1, None) == 0
+    assert lib.magma_dgetrf_batched(4, 0, None, 4, None, None, 1, None) == 0
+
+
+def test_getrs_batched_argument_codes(lib):
+    # src/zgetrs_batched.cpp:93-112
+    assert lib.magma_dgetrs_batched(999, 4, 1, None, 4, None, None, 4, 1, None) == -1
+    assert lib.magma_dgetrs_batched(111, -1, 1, None, 4, None, None, 4, 1, None) == -2
+    assert lib.magma_dgetrs_batched(111, 4, -1, None, 4, None, None, 4, 1, None) == -3
+    assert lib.magma_dgetrs_batched(111, 4, 1, None, 3, None, None, 4, 1, None) == -5
+    assert lib.magma_dgetrs_batched(111, 4, 1, None, 4, None, None, 3, 1, None) == -8
+    assert lib.magma_dgetrs_batched(111, 4, 0, None, 4, None, None, 4, 1, None) == 0
+
+
+def test_gesv_batched_argument_codes(lib):
+    # src/zgesv_batched.cpp:104-116
+    assert lib.magma_dgesv_batched(-1, 1, None, 4, None, None, 4, None, 1, None) == -1
+    assert lib.magma_dgesv_batched(4, -1, None, 4, None, None, 4, None, 1, None) == -2
+    assert lib.magma_dgesv_batched(4, 1, None, 3, None, None, 4, None, 1, None) == -4
+    assert lib.magma_dgesv_batched(4, 1, None, 4, None, None, 3, None, 1, None) == -6
+    assert lib.magma_dgesv_batched(0, 1, None, 1, None, None, 1, None, 1, None) == 0
+
+
+def test_strerror_and_nbparam(lib):
+    assert lib.magma_strerror(0) == b"success"
+    assert b"device memory" in lib.magma_strerror(-113)
+    assert lib.magma_strerror(-3) == b"invalid argument"
+    nb, rec = ctypes.c_int(), ctypes.c_int()
+    lib.magma_get_dgetrf_batched_nbparam(128, ctypes.byref(nb), ctypes.byref(rec))
+    assert nb.value > 0 and rec.value > 0
+
+
+def test_xerbla_message(lib, capfd):
+    lib.magma_xerbla(b"magma_dgetrf_batched", 4)
+    err = capfd.readouterr().err
+    assert "parameter 4 had an illegal value (info = -4)" in err
+
+
+def test_wtime_monotone(lib):
+    a = lib.magma_wtime()
+    b = lib.magma_wtime()
+    assert b >= a
