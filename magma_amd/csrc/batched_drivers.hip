@@ -17,6 +17,9 @@ magma_int_t getrf_batched_async(int m, int n, double** dA_array, int ldda, int**
         return getrf_tile_launch(n, dA_array, ldda, ipiv_array, info_array, batch, stream);
     if (getrf_lds_bytes(m, n) <= max_dynamic_lds())
         return getrf_lds_launch(m, n, dA_array, ldda, ipiv_array, info_array, batch, stream);
+    if (getrf_blocked_supported(m))
+        return getrf_blocked_launch(m, n, ldda, nullptr, nullptr, nullptr, m, n, dA_array, ipiv_array,
+                                    info_array, batch, stream);
     return MAGMA_ERR_NOT_SUPPORTED;
 }
 

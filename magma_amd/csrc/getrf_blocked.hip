@@ -1,0 +1,391 @@
+// Blocked right-looking batched LU for matrices too large for one workgroup's
+// on-chip storage, fixed or variable size. Per block column of width jb:
+//   1. panel:  LU with partial pivoting of the (m-k) x jb panel, rows spread
+//              over the workgroup's threads in registers;
+//   2. laswp:  the panel's interchanges applied to the columns left and right
+//              of the panel;
+//   3. trsm:   A12 := L11^{-1} A12 (unit lower);
+//   4. gemm:   A22 := A22 - A21 * A12, FP64 MFMA tiles.
+// Matrices whose size is exhausted exit at once, so one launch per step
+// serves a batch of mixed sizes.
+//
+// Reference structure: src/zgetrf_batched.cpp:149-203 (blocked loop),
+// src/zgetrf_panel_batched.cpp:102-196 (panel, laswp, trsm, gemm per step),
+// src/zgetrf_vbatched.cpp:20-124 (variable sizes, per-kernel early exit).
+// Pivot, scaling and zero-pivot semantics follow LAPACK dgetf2 as in
+// getrf_lds.hip.
+#include "magma_internal.h"
+#include "lu_device.h"
+#include "getrf_kernels.h"
+
+namespace mgi {
+
+constexpr int kPanelThreads = 256;
+
+struct SizeArgs {
+    int m, n, ld;                 // fixed-size values
+    const int* m_arr;             // per-matrix sizes (nullptr for fixed size)
+    const int* n_arr;
+    const int* ld_arr;
+    __device__ __forceinline__ void get(int b, int& mm, int& nn, int& ll) const
+    {
+        if (m_arr) { mm = m_arr[b]; nn = n_arr[b]; ll = ld_arr[b]; }
+        else { mm = m; nn = n; ll = ld; }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// 1. Panel: thread t owns panel rows t + 256*i (i < R); the jb <= NB columns
+//    of those rows sit in registers.
+// ---------------------------------------------------------------------------
+template <int R, int NB>
+__global__ void __launch_bounds__(kPanelThreads)
+panel_kernel(SizeArgs sz, int k, double** __restrict__ dA_array, int** __restrict__ ipiv_array,
+             int* __restrict__ info_array)
+{
+    __shared__ double s_val[kPanelThreads / kWave];
+    __shared__ int s_idx[kPanelThreads / kWave];
+    __shared__ double s_rowj[NB];
+    __shared__ double s_rowp[NB];
+
+    const int b = blockIdx.x;
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    if (k >= minmn) return;
+    const int jb = min(NB, minmn - k);
+    const int mloc = m - k;
+    const int t = threadIdx.x;
+    double* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
+    int* ipiv = ipiv_array[b];
+
+    double a[R][NB];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = t + kPanelThreads * i;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            a[i][j] = (r < mloc && j < jb) ? A[r + (size_t)j * ld] : 0.0;
+    }
+
+    int info = 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j >= jb) break;
+        // arg-max of |a(r, j)| over rows r >= j
+        double best = -1.0;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int r = t + kPanelThreads * i;
+            if (r >= j && r < mloc) {
+                double v = fabs(a[i][j]);
+                if (v > best) { best = v; bi = r; }
+            }
+        }
+        wave_argmax(best, bi);
+        const int w = t / kWave;
+        if ((t & (kWave - 1)) == 0) { s_val[w] = best; s_idx[w] = bi; }
+        __syncthreads();
+        best = s_val[0];
+        bi = s_idx[0];
+#pragma unroll
+        for (int q = 1; q < kPanelThreads / kWave; ++q) {
+            double ov = s_val[q];
+            int oi = s_idx[q];
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        const int p = bi;
+        // Row j (thread j, slot 0) and row p (thread p % 256, slot p / 256)
+        // publish their panel rows.
+        if (t == j) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) s_rowj[c] = a[0][c];
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (t + kPanelThreads * i == p) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) s_rowp[c] = a[i][c];
+            }
+        __syncthreads();
+        const double pv = s_rowp[j];
+        if (t == 0) ipiv[k + j] = k + p + 1;
+        if (pv != 0.0) {
+            if (p != j) {
+                if (t == j) {
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) a[0][c] = s_rowp[c];
+                }
+#pragma unroll
+                for (int i = 0; i < R; ++i)
+                    if (t + kPanelThreads * i == p) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) a[i][c] = s_rowj[c];
+                    }
+            }
+            const bool recip = fabs(pv) >= kSfminD;
+            const double rinv = 1.0 / pv;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = t + kPanelThreads * i;
+                if (r > j && r < mloc) {
+                    const double l = recip ? a[i][j] * rinv : a[i][j] / pv;
+                    a[i][j] = l;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+                        if (c > j && c < jb) a[i][c] = fma(-l, s_rowp[c], a[i][c]);
+                }
+            }
+        } else {
+            if (info == 0) info = k + j + 1;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                const int r = t + kPanelThreads * i;
+                if (r > j && r < mloc) {
+                    const double l = a[i][j];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c)
+                        if (c > j && c < jb) a[i][c] = fma(-l, s_rowp[c], a[i][c]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int r = t + kPanelThreads * i;
+        if (r < mloc) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (j < jb) A[r + (size_t)j * ld] = a[i][j];
+        }
+    }
+    if (t == 0 && info != 0 && info_array[b] == 0) info_array[b] = info;
+}
+
+// ---------------------------------------------------------------------------
+// 2. Row interchanges of panel k..k+jb-1 on every column outside the panel.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+laswp_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array, int** __restrict__ ipiv_array)
+{
+    const int b = blockIdx.y;
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    if (k >= minmn) return;
+    const int jb = min(nb, minmn - k);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int other = n - jb;                       // columns outside the panel
+    if (c >= other) return;
+    const int col = c < k ? c : c + jb;
+    double* A = dA_array[b] + (size_t)col * ld;
+    const int* ipiv = ipiv_array[b];
+    for (int j = k; j < k + jb; ++j) {
+        const int p = ipiv[j] - 1;
+        if (p != j && p > j && p < m) {
+            double t = A[j];
+            A[j] = A[p];
+            A[p] = t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. A12 := L11^{-1} A12, one thread per column of A12.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ void __launch_bounds__(256)
+trsm_kernel(SizeArgs sz, int k, double** __restrict__ dA_array)
+{
+    __shared__ double sL[NB * NB];
+    const int b = blockIdx.y;
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    if (k >= minmn) return;
+    const int jb = min(NB, minmn - k);
+    const int ncols = n - k - jb;
+    if (ncols <= 0) return;
+    const int c0 = blockIdx.x * blockDim.x;
+    if (c0 >= ncols) return;
+    const double* L = dA_array[b] + k + (size_t)k * ld;
+    for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
+        const int i = idx % jb, j = idx / jb;
+        sL[i + j * NB] = L[i + (size_t)j * ld];
+    }
+    __syncthreads();
+    const int c = c0 + threadIdx.x;
+    if (c >= ncols) return;
+    double* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
+    double x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = i < jb ? X[i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i > j && i < jb) x[i] = fma(-sL[i + j * NB], x[j], x[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i < jb) X[i] = x[i];
+}
+
+// ---------------------------------------------------------------------------
+// 4. A22 := A22 - A21 * A12 with v_mfma_f64_16x16x4f64. A workgroup of 4
+//    waves computes a 64x64 tile of A22 (each wave a 32x32 quarter = 2x2
+//    MFMA tiles); K = jb <= NB is staged in LDS.
+//
+//    MFMA f64 16x16x4 operand layout (lane l, 64 lanes):
+//      A (16x4):  A[l % 16][l / 16]
+//      B (4x16):  B[l / 16][l % 16]
+//      C (16x16): 4 values per lane, C[4*r + l/16][l % 16], r = 0..3
+// ---------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ void __launch_bounds__(256)
+gemm_update_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array)
+{
+    __shared__ double sA[NB][64 + 1];     // A21 tile, [kk][row]
+    __shared__ double sB[NB][64 + 1];     // A12 tile, [kk][col]
+    const int b = blockIdx.z;
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    if (k >= minmn) return;
+    const int jb = min(nb, minmn - k);
+    const int mr = m - k - jb, nc = n - k - jb;
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    if (r0 >= mr || c0 >= nc) return;
+    double* A = dA_array[b];
+    const double* A21 = A + (k + jb) + (size_t)k * ld;
+    const double* A12 = A + k + (size_t)(k + jb) * ld;
+    double* C = A + (k + jb) + (size_t)(k + jb) * ld;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < NB * 64; idx += 256) {
+        const int r = idx % 64, kk = idx / 64;           // A21: walk down columns
+        sA[kk][r] = (kk < jb && r0 + r < mr) ? A21[r0 + r + (size_t)kk * ld] : 0.0;
+        const int kb = idx % NB, cc = idx / NB;          // A12: walk down its short columns
+        sB[kb][cc] = (kb < jb && c0 + cc < nc) ? A12[kb + (size_t)(c0 + cc) * ld] : 0.0;
+    }
+    __syncthreads();
+    const int w = t / kWave, l = t % kWave;
+    const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+            acc[ti][tj] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 0; kb < NB; kb += 4) {
+        if (kb >= jb) break;
+        double af[2], bf[2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) af[ti] = sA[kb + l / 16][wr + 16 * ti + l % 16];
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) bf[tj] = sB[kb + l / 16][wc + 16 * tj + l % 16];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+            const int col = c0 + wc + 16 * tj + l % 16;
+            if (col >= nc) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + wr + 16 * ti + 4 * r + l / 16;
+                if (row < mr) {
+                    double* cp = C + row + (size_t)col * ld;
+                    *cp = *cp - acc[ti][tj][r];
+                }
+            }
+        }
+}
+
+__global__ void fill_int_kernel(int* p, int v, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct BlockedPlan {
+    int rows_per_thread;   // R
+    int nb;                // panel width
+};
+
+static bool plan_for(int max_m, BlockedPlan& p)
+{
+    if (max_m <= 256)       { p.rows_per_thread = 1; p.nb = 32; }
+    else if (max_m <= 512)  { p.rows_per_thread = 2; p.nb = 32; }
+    else if (max_m <= 1024) { p.rows_per_thread = 4; p.nb = 16; }
+    else if (max_m <= 2048) { p.rows_per_thread = 8; p.nb = 8; }
+    else return false;
+    return true;
+}
+
+template <int NB>
+static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n, int max_minmn,
+                                 double** dA_array, int** ipiv_array, int* info_array, int batch,
+                                 hipStream_t stream)
+{
+    for (int k = 0; k < max_minmn; k += NB) {
+        switch (R) {
+            case 1: panel_kernel<1, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 2: panel_kernel<2, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 4: panel_kernel<4, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 8: panel_kernel<8, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            default: return MAGMA_ERR_NOT_SUPPORTED;
+        }
+        MGI_LAUNCH_CHECK();
+        const int other = max_n;   // upper bound of the columns outside the panel
+        laswp_kernel<<<dim3(ceildiv(other, 256), batch), 256, 0, stream>>>(sz, k, NB, dA_array, ipiv_array);
+        MGI_LAUNCH_CHECK();
+        const int rest_n = max_n - k - 1;
+        const int rest_m = max_m - k - 1;
+        if (rest_n > 0) {
+            trsm_kernel<NB><<<dim3(ceildiv(rest_n, 256), batch), 256, 0, stream>>>(sz, k, dA_array);
+            MGI_LAUNCH_CHECK();
+            if (rest_m > 0) {
+                dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
+                gemm_update_kernel<NB><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array);
+                MGI_LAUNCH_CHECK();
+            }
+        }
+    }
+    return MAGMA_SUCCESS;
+}
+
+bool getrf_blocked_supported(int max_m) { BlockedPlan p; return plan_for(max_m, p); }
+
+magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const int* n_arr, const int* ld_arr,
+                                 int max_m, int max_n, double** dA_array, int** ipiv_array, int* info_array,
+                                 int batch, hipStream_t stream)
+{
+    BlockedPlan plan;
+    if (!plan_for(max_m, plan)) return MAGMA_ERR_NOT_SUPPORTED;
+    fill_int_kernel<<<ceildiv(batch, 256), 256, 0, stream>>>(info_array, 0, batch);
+    MGI_LAUNCH_CHECK();
+    SizeArgs sz{m, n, ldda, m_arr, n_arr, ld_arr};
+    const int max_minmn = std::min(max_m, max_n);
+    switch (plan.nb) {
+        case 32: return blocked_steps<32>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        case 16: return blocked_steps<16>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        case 8:  return blocked_steps<8>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        default: return MAGMA_ERR_NOT_SUPPORTED;
+    }
+}
+
+}  // namespace mgi

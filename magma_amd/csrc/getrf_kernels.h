@@ -20,6 +20,13 @@ size_t getrs_generic_bytes(int n);
 magma_int_t getrs_generic_launch(int trans, int n, int nrhs, double** dA_array, int ldda, int** ipiv_array,
                                  double** dB_array, int lddb, int batch, hipStream_t stream);
 
+// getrf_blocked.hip: blocked panel/laswp/trsm/gemm steps, fixed size (m_arr
+// == nullptr) or per-matrix sizes in device arrays; max_m <= 2048.
+bool getrf_blocked_supported(int max_m);
+magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const int* n_arr, const int* ld_arr,
+                                 int max_m, int max_n, double** dA_array, int** ipiv_array, int* info_array,
+                                 int batch, hipStream_t stream);
+
 // getrf_tile.hip: square n <= 128, register-tiled, MFMA FP64 trailing update.
 bool getrf_tile_supported(int n);
 magma_int_t getrf_tile_launch(int n, double** dA_array, int ldda, int** ipiv_array, int* info_array,

@@ -59,6 +59,18 @@ def test_getrf_square_random(mb, oracle, n):
     assert frac > 0.9
 
 
+@pytest.mark.parametrize("m,n", [(129, 129), (150, 150), (200, 200), (256, 256), (300, 300), (512, 512),
+                                 (300, 200), (200, 300), (700, 700), (1100, 1100)])
+def test_getrf_blocked_sizes(mb, oracle, m, n):
+    """Sizes beyond one workgroup's LDS take the blocked panel/laswp/trsm/gemm
+    path."""
+    rng = np.random.default_rng(m * 7 + n)
+    batch = 4 if m > 600 else 8
+    A = uniform_batch(rng, batch, m, cols=n)
+    LU, ipiv, info = run_getrf(mb, A, m=m)
+    check_getrf(oracle, A, LU, ipiv, info, m, n, f"{m}x{n}")
+
+
 @pytest.mark.parametrize("n,ld", [(20, 32), (100, 128), (128, 160)])
 def test_getrf_padded_ld(mb, oracle, n, ld):
     rng = np.random.default_rng(7 + n)
