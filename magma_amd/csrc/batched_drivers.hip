@@ -114,6 +114,9 @@ magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double** dA_arr
     if (n == 0 || nrhs == 0 || batchCount == 0) return 0;
     if (queue == nullptr) return MAGMA_ERR_NOT_INITIALIZED;
     hipSetDevice(queue->device);
+    if (mgi::gesv_tile_supported(n, nrhs))
+        return mgi::gesv_tile_launch(n, nrhs, dA_array, ldda, dipiv_array, dinfo_array, dB_array, lddb,
+                                     batchCount, queue->stream);
     info = mgi::getrf_batched_async(n, n, dA_array, ldda, dipiv_array, dinfo_array, batchCount, queue->stream);
     if (info != MAGMA_SUCCESS) return info;
     return mgi::getrs_batched_async(0, n, nrhs, dA_array, ldda, dipiv_array, dB_array, lddb,

@@ -31,5 +31,9 @@ magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const
 bool getrf_tile_supported(int n);
 magma_int_t getrf_tile_launch(int n, double** dA_array, int ldda, int** ipiv_array, int* info_array,
                               int batch, hipStream_t stream);
+// Fused LU + solve (A X = B) for n <= 128, nrhs <= 16.
+bool gesv_tile_supported(int n, int nrhs);
+magma_int_t gesv_tile_launch(int n, int nrhs, double** dA_array, int ldda, int** ipiv_array, int* info_array,
+                             double** dB_array, int lddb, int batch, hipStream_t stream);
 
 }  // namespace mgi

@@ -1,15 +1,544 @@
-// Square batched LU for n <= 128 with the matrix held in registers as 16x16
-// MFMA tiles (placeholder until the tiled kernel lands).
+// Square batched LU (and LU + solve) for n <= 128 with the whole matrix held
+// in registers as 16x16 FP64 tiles, one workgroup (4 waves) per matrix.
+//
+// Data layout. Tile (I,J) covers rows 16I..16I+15, columns 16J..16J+15 and is
+// owned by wave (I%2)*2 + (J%2). Within a tile, lane l holds
+//     element (16I + l%16, 16J + 4r + l/16) in register r = 0..3,
+// which is the accumulator layout of v_mfma_f64_16x16x4f64 for the transposed
+// tile (so an update C^T += U^T L^T lands in place) and lets 16 lanes read 128
+// contiguous bytes of one column when loading from column-major memory.
+//
+// Per 16-column panel P (rows k0 = 16P .. n-1):
+//   S1 owners of tile column P stage it in LDS;
+//   S2 one wave factors the panel: LAPACK pivoting (largest |a|, ties to the
+//      lowest current position), reciprocal scaling when |pivot| >= sfmin,
+//      rank-1 updates; rows are not moved during the panel -- their LAPACK
+//      positions are tracked and the panel is written back in position order;
+//   S3/S4 the panel's row permutation is applied to all other tile columns
+//      through an LDS staging area that holds only the rows that move;
+//   S5 U12 = L11^{-1} A12 by forward substitution, one thread per column;
+//   S6 A22 -= L21 U12 with v_mfma_f64_16x16x4f64 on the register tiles.
+// With a right-hand side (gesv), B sits in LDS as extra columns: it receives
+// the same interchanges, TRSM and updates (giving L^{-1} P^T B), then a blocked
+// back substitution with U produces X. A, B are read once; LU, X written once.
+//
+// Matrices with n not a multiple of 16 are padded to [A 0; 0 I] on chip; the
+// padding never becomes a pivot for a real column and is not written back.
+//
+// Reference behaviour: magmablas/zgetf2_devicefunc.cuh:218-294 (panel step),
+// magmablas/magma_templates.h:44-60 (ties), magmablas/zgetrf_batched_smallsq_noshfl.cu
+// (register LU with row tracking instead of physical swaps),
+// src/zgetrf_panel_batched.cpp:102-196 (panel / laswp / trsm / gemm order).
 #include "magma_internal.h"
+#include "lu_device.h"
 #include "getrf_kernels.h"
+
+#include <climits>
 
 namespace mgi {
 
-bool getrf_tile_supported(int n) { (void)n; return false; }
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
-magma_int_t getrf_tile_launch(int, double**, int, int**, int*, int, hipStream_t)
+constexpr int kTileThreads = 256;
+constexpr int kTileWaves = 4;
+
+__device__ __forceinline__ double wave_max_d(double v)
 {
-    return MAGMA_ERR_NOT_SUPPORTED;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, shfl_xor_d(v, off));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__device__ __forceinline__ int first_lane(unsigned long long mask) { return __ffsll((long long)mask) - 1; }
+
+template <int NT, bool WITH_B>
+struct TileSmem {
+    static constexpr int NP = 16 * NT;
+    static constexpr int LD16 = 17;                 // padded row length of 16-wide buffers
+    static constexpr int UCOLS = NP + (WITH_B ? 16 : 0);
+    static constexpr int ULD = UCOLS + 1;
+    static constexpr int SLD = NP + 1;
+    static constexpr int S_DOUBLES = 32 * SLD;
+    static constexpr int U_DOUBLES = 16 * ULD;
+    static constexpr int T_DOUBLES = (kTileWaves + 1) * 16 * LD16;
+    static constexpr int UNION_DOUBLES =
+        S_DOUBLES > U_DOUBLES ? (S_DOUBLES > T_DOUBLES ? S_DOUBLES : T_DOUBLES)
+                              : (U_DOUBLES > T_DOUBLES ? U_DOUBLES : T_DOUBLES);
+    double L[NP * LD16];                            // panel, position order
+    double U[UNION_DOUBLES];                        // laswp staging / U12 / solve scratch
+    double B[WITH_B ? NP * LD16 : 1];               // right-hand sides
+    int rowmap[NP];                                 // position -> source row (panel-relative)
+    int slotof[NP];                                 // moved row -> staging slot, or -1
+    int info;
+};
+
+template <int NT, bool WITH_B>
+__global__ void __launch_bounds__(kTileThreads, 2)
+getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int** __restrict__ ipiv_array,
+                  int* __restrict__ info_array, double** __restrict__ dB_array, int lddb)
+{
+    constexpr int NP = 16 * NT;
+    constexpr int NTW = (NT + 1) / 2;
+    constexpr int LD16 = TileSmem<NT, WITH_B>::LD16;
+    constexpr int ULD = TileSmem<NT, WITH_B>::ULD;
+    constexpr int SLD = TileSmem<NT, WITH_B>::SLD;
+    constexpr int UCOLS = TileSmem<NT, WITH_B>::UCOLS;
+    __shared__ TileSmem<NT, WITH_B> sm;
+
+    const int t = threadIdx.x;
+    const int w = t / kWave;
+    const int l = t % kWave;
+    const int wr = w >> 1, wc = w & 1;
+    const int lr = l % 16;            // row within a tile
+    const int lq = l / 16;            // column group within a tile
+    double* A = dA_array[blockIdx.x];
+    int* ipiv = ipiv_array[blockIdx.x];
+
+    // ---- load tiles (padding = identity) ----
+    double4_t acc[NTW][NTW];
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+        const int row = 16 * (2 * i + wr) + lr;
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * (2 * j + wc) + 4 * r + lq;
+                double v = 0.0;
+                if (2 * i + wr < NT && 2 * j + wc < NT) {
+                    if (row < n && col < n) v = A[row + (size_t)col * ldda];
+                    else v = (row == col) ? 1.0 : 0.0;
+                }
+                acc[i][j][r] = v;
+            }
+        }
+    }
+    if constexpr (WITH_B) {
+        const double* B = dB_array[blockIdx.x];
+        for (int idx = t; idx < NP * 16; idx += kTileThreads) {
+            const int c = idx / NP, row = idx % NP;
+            sm.B[row * LD16 + c] = (row < n && c < nrhs) ? B[row + (size_t)c * lddb] : 0.0;
+        }
+    }
+    if (t == 0) sm.info = 0;
+    __syncthreads();
+
+    const int npan = (n + 15) / 16;
+    for (int P = 0; P < npan; ++P) {
+        const int k0 = 16 * P;
+        const int mp = NP - k0;                       // panel rows (incl. padding)
+        const int pw = P % kTileWaves;                // wave that factors this panel
+
+        // ---- S1: stage tile column P (rows >= k0) into sm.L ----
+        if (wc == (P & 1)) {
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                const int I = 2 * i + wr;
+                if (I < NT && I >= P) {
+                    const int j = P >> 1;
+#pragma unroll
+                    for (int jj = 0; jj < NTW; ++jj) {
+                        if (jj != j) continue;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            sm.L[(16 * (I - P) + lr) * LD16 + 4 * r + lq] = acc[i][jj][r];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- S2: panel factorisation by one wave ----
+        if (w == pw) {
+            const bool has1 = mp > 64;
+            double x0[16], x1[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                x0[c] = (l < mp) ? sm.L[l * LD16 + c] : 0.0;
+                x1[c] = (has1 && l + 64 < mp) ? sm.L[(l + 64) * LD16 + c] : 0.0;
+            }
+            int pos0 = l, pos1 = l + 64;
+            bool done0 = !(l < mp), done1 = !(l + 64 < mp);
+            int info_local = 0;
+            const int creal = min(16, n - k0);        // real columns in this panel
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                // constant trip count (the body holds cross-lane operations,
+                // which rule out unrolling with an early exit)
+                if (c >= creal) continue;
+                const double v0 = done0 ? -1.0 : fabs(x0[c]);
+                const double v1 = done1 ? -1.0 : fabs(x1[c]);
+                const double m = wave_max_d(fmax(v0, v1));
+                unsigned long long b0 = __ballot(v0 == m);
+                unsigned long long b1 = __ballot(v1 == m);
+                if (__popcll(b0) + __popcll(b1) != 1) {
+                    // exact tie: LAPACK keeps the lowest current position
+                    const int cand = min(v0 == m ? pos0 : INT_MAX, v1 == m ? pos1 : INT_MAX);
+                    const int mpos = wave_min_i(cand);
+                    b0 = __ballot(v0 == m && pos0 == mpos);
+                    b1 = __ballot(v1 == m && pos1 == mpos);
+                }
+                const int ps = (b0 != 0ull) ? 0 : 1;
+                const int pl = (b0 != 0ull) ? first_lane(b0) : first_lane(b1);
+                const double pv = readlane_d(ps ? x1[c] : x0[c], pl);
+                const int pos_p = __builtin_amdgcn_readlane(ps ? pos1 : pos0, pl);
+                if (l == 0) ipiv[k0 + c] = k0 + pos_p + 1;
+                // LAPACK interchange of positions c and pos_p
+                const bool me0 = (ps == 0 && l == pl), me1 = (ps == 1 && l == pl);
+                pos0 = me0 ? c : (pos0 == c ? pos_p : pos0);
+                pos1 = me1 ? c : (pos1 == c ? pos_p : pos1);
+                done0 = done0 || me0;
+                done1 = done1 || me1;
+                double u[16];
+#pragma unroll
+                for (int c2 = 0; c2 < 16; ++c2)
+                    if (c2 > c) u[c2] = readlane_d(ps ? x1[c2] : x0[c2], pl);
+                double l0 = x0[c], l1 = x1[c];
+                if (pv != 0.0) {
+                    if (fabs(pv) >= kSfminD) {
+                        const double rinv = 1.0 / pv;
+                        l0 *= rinv;
+                        l1 *= rinv;
+                    } else {
+                        l0 /= pv;
+                        l1 /= pv;
+                    }
+                } else if (info_local == 0) {
+                    info_local = k0 + c + 1;
+                }
+                if (!done0) {
+                    x0[c] = l0;
+#pragma unroll
+                    for (int c2 = 0; c2 < 16; ++c2)
+                        if (c2 > c) x0[c2] = fma(-l0, u[c2], x0[c2]);
+                }
+                if (!done1) {
+                    x1[c] = l1;
+#pragma unroll
+                    for (int c2 = 0; c2 < 16; ++c2)
+                        if (c2 > c) x1[c2] = fma(-l1, u[c2], x1[c2]);
+                }
+            }
+            // Rows never chosen keep their positions only if untouched; the
+            // position map covers every panel row (padding included).
+            if (l < mp) {
+                sm.rowmap[pos0] = l;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) sm.L[pos0 * LD16 + c] = x0[c];
+            }
+            if (has1 && l + 64 < mp) {
+                sm.rowmap[pos1] = l + 64;
+#pragma unroll
+                for (int c = 0; c < 16; ++c) sm.L[pos1 * LD16 + c] = x1[c];
+            }
+            const bool mv0 = (l < mp) && pos0 != l;
+            const bool mv1 = (has1 && l + 64 < mp) && pos1 != l + 64;
+            const unsigned long long m0 = __ballot(mv0), m1 = __ballot(mv1);
+            const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+            if (l < mp) sm.slotof[l] = mv0 ? __popcll(m0 & below) : -1;
+            if (has1 && l + 64 < mp) sm.slotof[l + 64] = mv1 ? (__popcll(m0) + __popcll(m1 & below)) : -1;
+            if (l == 0 && info_local != 0 && sm.info == 0) sm.info = info_local;
+        }
+        __syncthreads();
+
+        // ---- S3: stage rows that move; reload the factored panel ----
+#pragma unroll
+        for (int i = 0; i < NTW; ++i) {
+            const int I = 2 * i + wr;
+            if (I >= NT || I < P) continue;
+            const int rr = 16 * (I - P) + lr;
+            const int slot = sm.slotof[rr];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int J = 2 * j + wc;
+                if (J >= NT) continue;
+                if (J == P) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.L[rr * LD16 + 4 * r + lq];
+                } else if (slot >= 0) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sm.U[slot * SLD + 16 * J + 4 * r + lq] = acc[i][j][r];
+                }
+            }
+        }
+        double btmp[WITH_B ? (NP * 16 + kTileThreads - 1) / kTileThreads : 1];
+        if constexpr (WITH_B) {
+#pragma unroll
+            for (int q = 0; q < (NP * 16 + kTileThreads - 1) / kTileThreads; ++q) {
+                const int idx = t + q * kTileThreads;
+                const int i = idx / 16, c = idx % 16;
+                if (i < mp) {
+                    const int src = sm.rowmap[i];
+                    btmp[q] = sm.B[(k0 + src) * LD16 + c];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- S4a: apply the row permutation from the staging area ----
+#pragma unroll
+        for (int i = 0; i < NTW; ++i) {
+            const int I = 2 * i + wr;
+            if (I >= NT || I < P) continue;
+            const int rr = 16 * (I - P) + lr;
+            const int src = sm.rowmap[rr];
+            if (src == rr) continue;
+            const int slot = sm.slotof[src];
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int J = 2 * j + wc;
+                if (J >= NT || J == P) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.U[slot * SLD + 16 * J + 4 * r + lq];
+            }
+        }
+        if constexpr (WITH_B) {
+#pragma unroll
+            for (int q = 0; q < (NP * 16 + kTileThreads - 1) / kTileThreads; ++q) {
+                const int idx = t + q * kTileThreads;
+                const int i = idx / 16, c = idx % 16;
+                if (i < mp) sm.B[(k0 + i) * LD16 + c] = btmp[q];
+            }
+        }
+        __syncthreads();
+
+        // ---- S4b: block row P right of the panel (and B) into sm.U ----
+        if (wr == (P & 1)) {
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int J = 2 * j + wc;
+                if (J >= NT || J <= P) continue;
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) {
+                    if (2 * i + wr != P) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = acc[i][j][r];
+                }
+            }
+        }
+        if constexpr (WITH_B) {
+            const int i = t / 16, c = t % 16;
+            sm.U[i * ULD + NP + c] = sm.B[(k0 + i) * LD16 + c];
+        }
+        __syncthreads();
+
+        // ---- S5: U12 = L11^{-1} A12, one thread per column ----
+        {
+            const int c0 = 16 * (P + 1);
+            const int col = c0 + t;
+            if (col < UCOLS) {
+                double xcol[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) xcol[i] = sm.U[i * ULD + col];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (i > c) xcol[i] = fma(-sm.L[i * LD16 + c], xcol[c], xcol[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) sm.U[i * ULD + col] = xcol[i];
+            }
+        }
+        __syncthreads();
+
+        // ---- S6: reload U12 tiles; trailing update with MFMA ----
+        if (wr == (P & 1)) {
+#pragma unroll
+            for (int j = 0; j < NTW; ++j) {
+                const int J = 2 * j + wc;
+                if (J >= NT || J <= P) continue;
+#pragma unroll
+                for (int i = 0; i < NTW; ++i) {
+                    if (2 * i + wr != P) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.U[lr * ULD + 16 * J + 4 * r + lq];
+                }
+            }
+        }
+        // negated L21 fragments of this wave's block rows: lane l holds
+        // -L(16I + l%16, 4kk + l/16)
+        double lf[NTW][4];
+#pragma unroll
+        for (int i = 0; i < NTW; ++i) {
+            const int I = 2 * i + wr;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk)
+                lf[i][kk] = (I < NT && I > P) ? -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const int J = 2 * j + wc;
+            if (J >= NT || J <= P) continue;
+            double uf[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) uf[kk] = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                const int I = 2 * i + wr;
+                if (I >= NT || I <= P) continue;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(uf[kk], lf[i][kk], acc[i][j], 0, 0, 0);
+            }
+        }
+        if constexpr (WITH_B) {
+            // B rows of the panel hold their final forward-substitution values.
+            {
+                const int i = t / 16, c = t % 16;
+                sm.B[(k0 + i) * LD16 + c] = sm.U[i * ULD + NP + c];
+            }
+            // B block rows below the panel: wave (I % 4) updates block row I.
+            double uf[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) uf[kk] = sm.U[(4 * kk + lq) * ULD + NP + lr];
+            for (int I = P + 1; I < NT; ++I) {
+                if ((I % kTileWaves) != w) continue;
+                double4_t cb;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cb[r] = sm.B[(16 * I + lr) * LD16 + 4 * r + lq];
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double lfr = -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq];
+                    cb = __builtin_amdgcn_mfma_f64_16x16x4f64(uf[kk], lfr, cb, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.B[(16 * I + lr) * LD16 + 4 * r + lq] = cb[r];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- write the factors ----
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+        const int I = 2 * i + wr;
+        const int row = 16 * I + lr;
+#pragma unroll
+        for (int j = 0; j < NTW; ++j) {
+            const int J = 2 * j + wc;
+            if (I >= NT || J >= NT) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * J + 4 * r + lq;
+                if (row < n && col < n) A[row + (size_t)col * ldda] = acc[i][j][r];
+            }
+        }
+    }
+    if (t == 0) info_array[blockIdx.x] = sm.info;
+
+    if constexpr (WITH_B) {
+        // ---- back substitution U X = Y, block rows bottom-up ----
+        double* T = sm.U;                         // [5][16][LD16]: 4 partials + U_II
+        const int nb_real = (n + 15) / 16;
+        for (int I = nb_real - 1; I >= 0; --I) {
+            double4_t part = {0.0, 0.0, 0.0, 0.0};
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                if (2 * i + wr != I) continue;
+#pragma unroll
+                for (int j = 0; j < NTW; ++j) {
+                    const int J = 2 * j + wc;
+                    if (J >= NT || J <= I) continue;
+                    any = true;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const double xb = sm.B[(16 * J + 4 * r + lq) * LD16 + lr];
+                        part = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[i][j][r], xb, part, 0, 0, 0);
+                    }
+                }
+                // diagonal tile to the solve scratch
+#pragma unroll
+                for (int j = 0; j < NTW; ++j) {
+                    if (2 * j + wc != I) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) T[(4 * 16 + lr) * LD16 + 4 * r + lq] = acc[i][j][r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) T[(w * 16 + 4 * r + lq) * LD16 + lr] = any ? part[r] : 0.0;
+            __syncthreads();
+            if (t < 16) {
+                const int c = t;
+                double y[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    double s = sm.B[(16 * I + i) * LD16 + c];
+#pragma unroll
+                    for (int q = 0; q < kTileWaves; ++q) s -= T[(q * 16 + i) * LD16 + c];
+                    y[i] = s;
+                }
+#pragma unroll
+                for (int i = 15; i >= 0; --i) {
+                    y[i] = y[i] / T[(64 + i) * LD16 + i];
+#pragma unroll
+                    for (int i2 = 0; i2 < 16; ++i2)
+                        if (i2 < i) y[i2] = fma(-T[(64 + i2) * LD16 + i], y[i], y[i2]);
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) sm.B[(16 * I + i) * LD16 + c] = y[i];
+            }
+            __syncthreads();
+        }
+        double* B = dB_array[blockIdx.x];
+        for (int idx = t; idx < n * nrhs; idx += kTileThreads) {
+            const int row = idx % n, c = idx / n;
+            B[row + (size_t)c * lddb] = sm.B[row * LD16 + c];
+        }
+    }
+}
+
+bool getrf_tile_supported(int n) { return n >= 1 && n <= 128; }
+
+template <bool WITH_B>
+static magma_int_t launch_nt(int n, int nrhs, double** dA_array, int ldda, int** ipiv_array, int* info_array,
+                             double** dB_array, int lddb, int batch, hipStream_t stream)
+{
+    const int nt = (n + 15) / 16;
+    dim3 grid(batch), block(kTileThreads);
+#define MGI_TILE_CASE(NTV)                                                                              \
+    case NTV:                                                                                           \
+        getrf_tile_kernel<NTV, WITH_B><<<grid, block, 0, stream>>>(n, nrhs, dA_array, ldda, ipiv_array, \
+                                                                   info_array, dB_array, lddb);         \
+        break;
+    switch (nt) {
+        MGI_TILE_CASE(1)
+        MGI_TILE_CASE(2)
+        MGI_TILE_CASE(3)
+        MGI_TILE_CASE(4)
+        MGI_TILE_CASE(5)
+        MGI_TILE_CASE(6)
+        MGI_TILE_CASE(7)
+        MGI_TILE_CASE(8)
+        default: return MAGMA_ERR_NOT_SUPPORTED;
+    }
+#undef MGI_TILE_CASE
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+magma_int_t getrf_tile_launch(int n, double** dA_array, int ldda, int** ipiv_array, int* info_array,
+                              int batch, hipStream_t stream)
+{
+    if (!getrf_tile_supported(n)) return MAGMA_ERR_NOT_SUPPORTED;
+    return launch_nt<false>(n, 0, dA_array, ldda, ipiv_array, info_array, nullptr, 0, batch, stream);
+}
+
+bool gesv_tile_supported(int n, int nrhs) { return getrf_tile_supported(n) && nrhs >= 1 && nrhs <= 16; }
+
+magma_int_t gesv_tile_launch(int n, int nrhs, double** dA_array, int ldda, int** ipiv_array, int* info_array,
+                             double** dB_array, int lddb, int batch, hipStream_t stream)
+{
+    if (!gesv_tile_supported(n, nrhs)) return MAGMA_ERR_NOT_SUPPORTED;
+    return launch_nt<true>(n, nrhs, dA_array, ldda, ipiv_array, info_array, dB_array, lddb, batch, stream);
 }
 
 }  // namespace mgi
