@@ -74,6 +74,8 @@ struct TileSmem {
     double L[NP * LD16];                            // panel, position order
     double U[UNION_DOUBLES];                        // laswp staging / U12 / solve scratch
     double B[WITH_B ? NP * LD16 : 1];               // right-hand sides
+    double Li[16 * LD16];                           // L11^{-1} of the current panel
+    double prow[16];                                // pivot row broadcast
     int rowmap[NP];                                 // position -> source row (panel-relative)
     int slotof[NP];                                 // moved row -> staging slot, or -1
     int info;
@@ -156,6 +158,8 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
         __syncthreads();
 
         // ---- S2: panel factorisation by one wave ----
+        // Lane l holds panel rows l and l+64 (the second only while the panel
+        // is taller than 64 rows). A row leaves the candidate set once chosen.
         if (w == pw) {
             const bool has1 = mp > 64;
             double x0[16], x1[16];
@@ -165,7 +169,7 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
                 x1[c] = (has1 && l + 64 < mp) ? sm.L[(l + 64) * LD16 + c] : 0.0;
             }
             int pos0 = l, pos1 = l + 64;
-            bool done0 = !(l < mp), done1 = !(l + 64 < mp);
+            bool alive0 = l < mp, alive1 = has1 && (l + 64 < mp);
             int info_local = 0;
             const int creal = min(16, n - k0);        // real columns in this panel
 #pragma unroll
@@ -173,33 +177,47 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
                 // constant trip count (the body holds cross-lane operations,
                 // which rule out unrolling with an early exit)
                 if (c >= creal) continue;
-                const double v0 = done0 ? -1.0 : fabs(x0[c]);
-                const double v1 = done1 ? -1.0 : fabs(x1[c]);
-                const double m = wave_max_d(fmax(v0, v1));
+                const double v0 = alive0 ? fabs(x0[c]) : -1.0;
+                const double v1 = alive1 ? fabs(x1[c]) : -1.0;
+                const double m = wave_max_dpp(fmax(v0, v1));
                 unsigned long long b0 = __ballot(v0 == m);
                 unsigned long long b1 = __ballot(v1 == m);
                 if (__popcll(b0) + __popcll(b1) != 1) {
                     // exact tie: LAPACK keeps the lowest current position
                     const int cand = min(v0 == m ? pos0 : INT_MAX, v1 == m ? pos1 : INT_MAX);
-                    const int mpos = wave_min_i(cand);
+                    const int mpos = wave_min_dpp(cand);
                     b0 = __ballot(v0 == m && pos0 == mpos);
                     b1 = __ballot(v1 == m && pos1 == mpos);
                 }
                 const int ps = (b0 != 0ull) ? 0 : 1;
                 const int pl = (b0 != 0ull) ? first_lane(b0) : first_lane(b1);
-                const double pv = readlane_d(ps ? x1[c] : x0[c], pl);
+                // the pivot row goes to LDS; every lane reads it back
+                if (ps == 0) {
+                    if (l == pl) {
+#pragma unroll
+                        for (int c2 = 0; c2 < 16; ++c2)
+                            if (c2 >= c) sm.prow[c2] = x0[c2];
+                    }
+                } else {
+                    if (l == pl) {
+#pragma unroll
+                        for (int c2 = 0; c2 < 16; ++c2)
+                            if (c2 >= c) sm.prow[c2] = x1[c2];
+                    }
+                }
                 const int pos_p = __builtin_amdgcn_readlane(ps ? pos1 : pos0, pl);
                 if (l == 0) ipiv[k0 + c] = k0 + pos_p + 1;
                 // LAPACK interchange of positions c and pos_p
                 const bool me0 = (ps == 0 && l == pl), me1 = (ps == 1 && l == pl);
                 pos0 = me0 ? c : (pos0 == c ? pos_p : pos0);
                 pos1 = me1 ? c : (pos1 == c ? pos_p : pos1);
-                done0 = done0 || me0;
-                done1 = done1 || me1;
+                alive0 = alive0 && !me0;
+                alive1 = alive1 && !me1;
                 double u[16];
 #pragma unroll
                 for (int c2 = 0; c2 < 16; ++c2)
-                    if (c2 > c) u[c2] = readlane_d(ps ? x1[c2] : x0[c2], pl);
+                    if (c2 >= c) u[c2] = sm.prow[c2];
+                const double pv = u[c];
                 double l0 = x0[c], l1 = x1[c];
                 if (pv != 0.0) {
                     if (fabs(pv) >= kSfminD) {
@@ -213,21 +231,21 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
                 } else if (info_local == 0) {
                     info_local = k0 + c + 1;
                 }
-                if (!done0) {
-                    x0[c] = l0;
+                // rows still alive take the multiplier; others stay as they are
+                x0[c] = alive0 ? l0 : x0[c];
+                const double f0 = alive0 ? l0 : 0.0;
+#pragma unroll
+                for (int c2 = 0; c2 < 16; ++c2)
+                    if (c2 > c) x0[c2] = fma(-f0, u[c2], x0[c2]);
+                if (has1) {
+                    x1[c] = alive1 ? l1 : x1[c];
+                    const double f1 = alive1 ? l1 : 0.0;
 #pragma unroll
                     for (int c2 = 0; c2 < 16; ++c2)
-                        if (c2 > c) x0[c2] = fma(-l0, u[c2], x0[c2]);
-                }
-                if (!done1) {
-                    x1[c] = l1;
-#pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2)
-                        if (c2 > c) x1[c2] = fma(-l1, u[c2], x1[c2]);
+                        if (c2 > c) x1[c2] = fma(-f1, u[c2], x1[c2]);
                 }
             }
-            // Rows never chosen keep their positions only if untouched; the
-            // position map covers every panel row (padding included).
+            // every panel row (padding included) to its LAPACK position
             if (l < mp) {
                 sm.rowmap[pos0] = l;
 #pragma unroll
@@ -245,6 +263,34 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
             if (l < mp) sm.slotof[l] = mv0 ? __popcll(m0 & below) : -1;
             if (has1 && l + 64 < mp) sm.slotof[l + 64] = mv1 ? (__popcll(m0) + __popcll(m1 & below)) : -1;
             if (l == 0 && info_local != 0 && sm.info == 0) sm.info = info_local;
+
+            // L11^{-1} by forward substitution on the unit columns: lane
+            // 4j + q owns column j, rows q, q+4, q+8, q+12; the solved entry
+            // of step k is broadcast within the quad.
+            {
+                const int j = l >> 2, q = l & 3;
+                double xi[4];
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) xi[s4] = (q + 4 * s4 == j) ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < 15; ++k) {
+                    double xk;
+                    switch (k & 3) {
+                        case 0: xk = quad_bcast_d<0>(xi[k >> 2]); break;
+                        case 1: xk = quad_bcast_d<1>(xi[k >> 2]); break;
+                        case 2: xk = quad_bcast_d<2>(xi[k >> 2]); break;
+                        default: xk = quad_bcast_d<3>(xi[k >> 2]); break;
+                    }
+#pragma unroll
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        const int i = q + 4 * s4;
+                        const double coef = (i > k) ? sm.L[i * LD16 + k] : 0.0;
+                        xi[s4] = fma(-coef, xk, xi[s4]);
+                    }
+                }
+#pragma unroll
+                for (int s4 = 0; s4 < 4; ++s4) sm.Li[(q + 4 * s4) * LD16 + j] = xi[s4];
+            }
         }
         __syncthreads();
 
@@ -329,28 +375,12 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
         }
         __syncthreads();
 
-        // ---- S5: U12 = L11^{-1} A12, one thread per column ----
-        {
-            const int c0 = 16 * (P + 1);
-            const int col = c0 + t;
-            if (col < UCOLS) {
-                double xcol[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) xcol[i] = sm.U[i * ULD + col];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-#pragma unroll
-                    for (int i = 0; i < 16; ++i)
-                        if (i > c) xcol[i] = fma(-sm.L[i * LD16 + c], xcol[c], xcol[i]);
-                }
-#pragma unroll
-                for (int i = 0; i < 16; ++i) sm.U[i * ULD + col] = xcol[i];
-            }
-        }
-        __syncthreads();
-
-        // ---- S6: reload U12 tiles; trailing update with MFMA ----
+        // ---- S5: U12 = L11^{-1} A12 with MFMA on the owners of block row P ----
+        // (U12^T = A12^T L11^{-T}: the result lands in the tile layout.)
         if (wr == (P & 1)) {
+            double lb[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) lb[kk] = sm.Li[lr * LD16 + 4 * kk + lq];
 #pragma unroll
             for (int j = 0; j < NTW; ++j) {
                 const int J = 2 * j + wc;
@@ -358,11 +388,38 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
 #pragma unroll
                 for (int i = 0; i < NTW; ++i) {
                     if (2 * i + wr != P) continue;
+                    double4_t uacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.U[lr * ULD + 16 * J + 4 * r + lq];
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const double a12 = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
+                        uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lb[kk], uacc, 0, 0, 0);
+                    }
+                    acc[i][j] = uacc;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = uacc[r];
                 }
             }
         }
+        if constexpr (WITH_B) {
+            // the right-hand sides' panel rows, on a wave outside block row P
+            if (w == (((P & 1) ^ 1) << 1)) {
+                double4_t uacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double a12 = sm.U[(4 * kk + lq) * ULD + NP + lr];
+                    const double lbk = sm.Li[lr * LD16 + 4 * kk + lq];
+                    uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lbk, uacc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    sm.U[lr * ULD + NP + 4 * r + lq] = uacc[r];
+                    sm.B[(k0 + lr) * LD16 + 4 * r + lq] = uacc[r];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- S6: trailing update with MFMA ----
         // negated L21 fragments of this wave's block rows: lane l holds
         // -L(16I + l%16, 4kk + l/16)
         double lf[NTW][4];
@@ -390,11 +447,6 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
             }
         }
         if constexpr (WITH_B) {
-            // B rows of the panel hold their final forward-substitution values.
-            {
-                const int i = t / 16, c = t % 16;
-                sm.B[(k0 + i) * LD16 + c] = sm.U[i * ULD + NP + c];
-            }
             // B block rows below the panel: wave (I % 4) updates block row I.
             double uf[4];
 #pragma unroll
@@ -466,25 +518,40 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
 #pragma unroll
             for (int r = 0; r < 4; ++r) T[(w * 16 + 4 * r + lq) * LD16 + lr] = any ? part[r] : 0.0;
             __syncthreads();
-            if (t < 16) {
-                const int c = t;
-                double y[16];
+            if (w == 0) {
+                // lane 4c + q: right-hand side c, rows q, q+4, q+8, q+12 of the
+                // block; the solved entry of row k is broadcast within the quad
+                const int c = l >> 2, q = l & 3;
+                double y[4], rd[4];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    double s = sm.B[(16 * I + i) * LD16 + c];
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int i = q + 4 * s4;
+                    double v = sm.B[(16 * I + i) * LD16 + c];
 #pragma unroll
-                    for (int q = 0; q < kTileWaves; ++q) s -= T[(q * 16 + i) * LD16 + c];
-                    y[i] = s;
+                    for (int qq = 0; qq < kTileWaves; ++qq) v -= T[(qq * 16 + i) * LD16 + c];
+                    y[s4] = v;
+                    rd[s4] = 1.0 / T[(64 + i) * LD16 + i];
                 }
 #pragma unroll
-                for (int i = 15; i >= 0; --i) {
-                    y[i] = y[i] / T[(64 + i) * LD16 + i];
+                for (int k = 15; k >= 0; --k) {
+                    const int sk = k >> 2;
+                    y[sk] = (q == (k & 3)) ? y[sk] * rd[sk] : y[sk];
+                    double xk;
+                    switch (k & 3) {
+                        case 0: xk = quad_bcast_d<0>(y[sk]); break;
+                        case 1: xk = quad_bcast_d<1>(y[sk]); break;
+                        case 2: xk = quad_bcast_d<2>(y[sk]); break;
+                        default: xk = quad_bcast_d<3>(y[sk]); break;
+                    }
 #pragma unroll
-                    for (int i2 = 0; i2 < 16; ++i2)
-                        if (i2 < i) y[i2] = fma(-T[(64 + i2) * LD16 + i], y[i], y[i2]);
+                    for (int s4 = 0; s4 < 4; ++s4) {
+                        const int i = q + 4 * s4;
+                        const double coef = (i < k) ? T[(64 + i) * LD16 + k] : 0.0;
+                        y[s4] = fma(-coef, xk, y[s4]);
+                    }
                 }
 #pragma unroll
-                for (int i = 0; i < 16; ++i) sm.B[(16 * I + i) * LD16 + c] = y[i];
+                for (int s4 = 0; s4 < 4; ++s4) sm.B[(16 * I + q + 4 * s4) * LD16 + c] = y[s4];
             }
             __syncthreads();
         }

@@ -61,3 +61,68 @@ __device__ __forceinline__ void wave_argmax_f(float& v, int& idx)
 }
 
 }  // namespace mgi
+
+namespace mgi {
+
+// ---------------------------------------------------------------------------
+// DPP (data-parallel primitives) cross-lane moves for gfx9-family waves.
+// Control words: quad_perm [a,b,c,d] = a | b<<2 | c<<4 | d<<6; row_half_mirror
+// 0x141, row_mirror 0x140, row_bcast15 0x142, row_bcast31 0x143.
+// ---------------------------------------------------------------------------
+constexpr int kDppQuadXor1 = 0xB1;      // [1,0,3,2]
+constexpr int kDppQuadXor2 = 0x4E;      // [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+constexpr int kDppBcast15 = 0x142;
+constexpr int kDppBcast31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Maximum over the 64 lanes, returned as a wave-uniform value. Within each
+// 16-lane row: quad xor 1, quad xor 2, half-mirror, mirror; across rows:
+// broadcast of lane 15 into rows 1 and 3, then of lane 31 into rows 2 and 3,
+// leaving the total in lane 63.
+__device__ __forceinline__ double wave_max_dpp(double v)
+{
+    v = fmax(v, dpp_d<kDppQuadXor1>(v));
+    v = fmax(v, dpp_d<kDppQuadXor2>(v));
+    v = fmax(v, dpp_d<kDppHalfMirror>(v));
+    v = fmax(v, dpp_d<kDppMirror>(v));
+    v = fmax(v, dpp_d<kDppBcast15, 0xA>(v));
+    v = fmax(v, dpp_d<kDppBcast31, 0xC>(v));
+    return readlane_d(v, 63);
+}
+
+__device__ __forceinline__ int wave_min_dpp(int v)
+{
+    v = min(v, dpp_i<kDppQuadXor1>(v));
+    v = min(v, dpp_i<kDppQuadXor2>(v));
+    v = min(v, dpp_i<kDppHalfMirror>(v));
+    v = min(v, dpp_i<kDppMirror>(v));
+    v = min(v, dpp_i<kDppBcast15, 0xA>(v));
+    v = min(v, dpp_i<kDppBcast31, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Broadcast lane Q of every quad to the whole quad.
+template <int Q>
+__device__ __forceinline__ double quad_bcast_d(double v)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    return dpp_d<ctrl>(v);
+}
+
+}  // namespace mgi
