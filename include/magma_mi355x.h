@@ -199,6 +199,49 @@ magma_int_t magma_dgesv_vbatched(magma_int_t *n, magma_int_t *nrhs,
         double **dB_array, magma_int_t *lddb, magma_int_t *dinfo_array,
         magma_int_t batchCount, magma_queue_t queue);
 
+/* Variable-size plans (no MAGMA counterpart). A plan groups a batch into
+ * n-buckets once (sizes given on the host) and keeps the device index maps,
+ * so executing it allocates nothing and never synchronises; it can be
+ * recorded once into a hipGraph and replayed. h_variants (nullable) holds a
+ * kernel choice per bucket id (0 auto, 1 register-tile, 2/3/4 blocked with
+ * panel width 32/16/8), e.g. from magma_amd.autotune. h_nrhs == NULL makes
+ * a factor-only plan. */
+struct magma_vbatched_plan;
+typedef struct magma_vbatched_plan* magma_vbatched_plan_t;
+magma_int_t magma_dgesv_vbatched_plan_create(const magma_int_t *h_n, const magma_int_t *h_nrhs,
+        magma_int_t batchCount, const magma_int_t *h_variants, magma_queue_t queue,
+        magma_vbatched_plan_t *plan);
+magma_int_t magma_vbatched_plan_num_buckets(magma_vbatched_plan_t plan);
+/* info5 = {bucket id (-1: rectangular group), variant, count, max_n, max_nrhs} */
+magma_int_t magma_vbatched_plan_bucket(magma_vbatched_plan_t plan, magma_int_t i, magma_int_t *info5);
+magma_int_t magma_dgesv_vbatched_plan_exec(magma_vbatched_plan_t plan, magma_int_t *n, magma_int_t *nrhs,
+        double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+        double **dB_array, magma_int_t *lddb, magma_int_t *dinfo_array, magma_queue_t queue);
+magma_int_t magma_dgesv_vbatched_plan_capture(magma_vbatched_plan_t plan, magma_int_t *n, magma_int_t *nrhs,
+        double **dA_array, magma_int_t *ldda, magma_int_t **dipiv_array,
+        double **dB_array, magma_int_t *lddb, magma_int_t *dinfo_array, magma_queue_t queue);
+magma_int_t magma_vbatched_plan_replay(magma_vbatched_plan_t plan, magma_queue_t queue);
+void magma_vbatched_plan_destroy(magma_vbatched_plan_t plan);
+
+/* ------------------------------------------------------------------------- */
+/* Single precision batched LU: the "s" codegen forms of the prototypes at    */
+/* include/magma_zbatched.h:848, :464 and :1009 (include/magma_sbatched.h).   */
+/* ------------------------------------------------------------------------- */
+magma_int_t magma_sgetrf_batched(magma_int_t m, magma_int_t n, float **dA_array, magma_int_t ldda,
+                                 magma_int_t **ipiv_array, magma_int_t *info_array,
+                                 magma_int_t batchCount, magma_queue_t queue);
+magma_int_t magma_sgetrs_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                                 float **dA_array, magma_int_t ldda, magma_int_t **dipiv_array,
+                                 float **dB_array, magma_int_t lddb,
+                                 magma_int_t batchCount, magma_queue_t queue);
+magma_int_t magma_sgesv_batched(magma_int_t n, magma_int_t nrhs, float **dA_array, magma_int_t ldda,
+                                magma_int_t **dipiv_array, float **dB_array, magma_int_t lddb,
+                                magma_int_t *dinfo_array, magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zbatched.h:28, s form */
+void magma_sset_pointer(float **output_array, float *input, magma_int_t lda,
+                        magma_int_t row, magma_int_t column, magma_int_t batch_offset,
+                        magma_int_t batchCount, magma_queue_t queue);
+
 /* ------------------------------------------------------------------------- */
 /* Mixed-precision iterative refinement                                        */
 /* ------------------------------------------------------------------------- */

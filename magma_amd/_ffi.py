@@ -74,6 +74,22 @@ _PROTOS = {
                                       c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "magma_dgesv_vbatched": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "magma_sset_pointer": (None, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p]),
+    "magma_sgetrf_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "magma_sgetrs_batched": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                                     c_int, c_void_p]),
+    "magma_sgesv_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                    c_int, c_void_p]),
+    "magma_dgesv_vbatched_plan_create": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                                 ctypes.POINTER(c_void_p)]),
+    "magma_vbatched_plan_num_buckets": (c_int, [c_void_p]),
+    "magma_vbatched_plan_bucket": (c_int, [c_void_p, c_int, c_void_p]),
+    "magma_dgesv_vbatched_plan_exec": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                               c_void_p, c_void_p, c_void_p, c_void_p]),
+    "magma_dgesv_vbatched_plan_capture": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                  c_void_p, c_void_p, c_void_p, c_void_p]),
+    "magma_vbatched_plan_replay": (c_int, [c_void_p, c_void_p]),
+    "magma_vbatched_plan_destroy": (None, [c_void_p]),
     "magma_dhgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
     "magma_dbgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,

@@ -171,3 +171,238 @@ def dgesv_batched(A: torch.Tensor, B: torch.Tensor, ipiv: torch.Tensor | None = 
     if rc < 0:
         raise MagmaError("magma_dgesv_batched", rc)
     return ipiv, info
+
+
+# ---------------------------------------------------------------------------
+# Single precision
+# ---------------------------------------------------------------------------
+def sgetrf_batched(A: torch.Tensor, m: int | None = None, ipiv: torch.Tensor | None = None,
+                   info: torch.Tensor | None = None, queue: Queue | None = None):
+    """FP32 form of dgetrf_batched."""
+    _check_batch(A, "A", torch.float32)
+    batch, n, ldda = A.shape
+    m = ldda if m is None else m
+    queue = queue or default_queue(A.device)
+    k = max(min(m, n), 1)
+    ipiv = ipiv if ipiv is not None else torch.zeros((batch, k), dtype=torch.int32, device=A.device)
+    info = info if info is not None else torch.zeros(batch, dtype=torch.int32, device=A.device)
+    lib = _ffi.load()
+    pA = _sptr(A, queue)
+    pP = pointer_array(_ipiv_view(ipiv), queue)
+    rc = lib.magma_sgetrf_batched(m, n, pA.data_ptr(), ldda, pP.data_ptr(), info.data_ptr(), batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_sgetrf_batched", rc)
+    return ipiv, info
+
+
+def sgetrs_batched(trans: int, LU: torch.Tensor, ipiv: torch.Tensor, B: torch.Tensor,
+                   queue: Queue | None = None) -> None:
+    _check_batch(LU, "LU", torch.float32)
+    _check_batch(B, "B", torch.float32)
+    batch, n, ldda = LU.shape
+    _, nrhs, lddb = B.shape
+    queue = queue or default_queue(LU.device)
+    lib = _ffi.load()
+    # keep the pointer arrays alive until the launches are queued
+    pA, pP, pB = _sptr(LU, queue), pointer_array(_ipiv_view(ipiv), queue), _sptr(B, queue)
+    rc = lib.magma_sgetrs_batched(trans, n, nrhs, pA.data_ptr(), ldda, pP.data_ptr(), pB.data_ptr(), lddb,
+                                  batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_sgetrs_batched", rc)
+
+
+def _sptr(t: torch.Tensor, queue: Queue) -> torch.Tensor:
+    lib = _ffi.load()
+    batch = t.shape[0]
+    ptrs = torch.empty(max(batch, 1), dtype=torch.int64, device=t.device)
+    if batch:
+        lib.magma_sset_pointer(ptrs.data_ptr(), t.data_ptr(), t.shape[2], 0, 0, t.shape[1] * t.shape[2], batch,
+                               queue.handle)
+    return ptrs
+
+
+# ---------------------------------------------------------------------------
+# Variable size
+# ---------------------------------------------------------------------------
+class VBatch:
+    """A batch of column-major matrices of different sizes packed into one
+    flat device buffer (matrix i at element offset offs[i], leading dimension
+    ld[i]), with the device pointer and size arrays MAGMA's vbatched calls
+    take."""
+
+    def __init__(self, m, n, ld=None, dtype=torch.float64, device=None, cols=None):
+        import numpy as np
+        self.m = np.asarray(m, dtype=np.int32)
+        self.n = np.asarray(n, dtype=np.int32)
+        self.ld = np.maximum(self.m, 1).astype(np.int32) if ld is None else np.asarray(ld, dtype=np.int32)
+        ncols = self.n if cols is None else np.asarray(cols, dtype=np.int32)
+        self.cols = ncols
+        sizes = self.ld.astype(np.int64) * ncols.astype(np.int64)
+        self.offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        self.dtype = dtype
+        self.data = torch.zeros(int(sizes.sum()) + 1, dtype=dtype, device=device)
+        elem = self.data.element_size()
+        base = self.data.data_ptr()
+        self.ptrs = torch.from_numpy(base + self.offs * elem).to(device)
+        self.d_m = torch.from_numpy(self.m).to(device)
+        self.d_n = torch.from_numpy(self.n).to(device)
+        self.d_ld = torch.from_numpy(self.ld).to(device)
+
+    def __len__(self):
+        return len(self.m)
+
+    def set(self, i, M):
+        """Store row-major numpy matrix M (m_i x cols_i) into slot i."""
+        import numpy as np
+        m, c, ld, off = int(self.m[i]), int(self.cols[i]), int(self.ld[i]), int(self.offs[i])
+        buf = np.zeros((c, ld))
+        buf[:, :m] = np.asarray(M).T
+        self.data[off:off + c * ld] = torch.from_numpy(buf.reshape(-1)).to(self.data.device, self.dtype)
+
+    def get(self, i):
+        m, c, ld, off = int(self.m[i]), int(self.cols[i]), int(self.ld[i]), int(self.offs[i])
+        return self.data[off:off + c * ld].view(c, ld)[:, :m].T.cpu().numpy()
+
+
+class IpivV:
+    """Per-matrix ipiv buffers for a variable-size batch."""
+
+    def __init__(self, k, device=None):
+        import numpy as np
+        k = np.maximum(np.asarray(k, dtype=np.int64), 1)
+        self.k = k
+        self.offs = np.concatenate([[0], np.cumsum(k)[:-1]]).astype(np.int64)
+        device = device or torch.device("cuda", torch.cuda.current_device())
+        self.data = torch.zeros(int(k.sum()), dtype=torch.int32, device=device)
+        self.ptrs = torch.from_numpy(self.data.data_ptr() + self.offs * 4).to(device)
+
+    def get(self, i, count):
+        off = int(self.offs[i])
+        return self.data[off:off + count].cpu().numpy()
+
+
+def dgetrf_vbatched(A: VBatch, ipiv: IpivV | None = None, info: torch.Tensor | None = None,
+                    queue: Queue | None = None):
+    import numpy as np
+    queue = queue or default_queue(A.data.device)
+    ipiv = ipiv or IpivV(np.minimum(A.m, A.n), A.data.device)
+    info = info if info is not None else torch.zeros(len(A), dtype=torch.int32, device=A.data.device)
+    rc = _ffi.load().magma_dgetrf_vbatched(A.d_m.data_ptr(), A.d_n.data_ptr(), A.ptrs.data_ptr(),
+                                           A.d_ld.data_ptr(), ipiv.ptrs.data_ptr(), info.data_ptr(), len(A),
+                                           queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgetrf_vbatched", rc)
+    return ipiv, info
+
+
+def dgetrf_vbatched_max_nocheck(A: VBatch, ipiv: IpivV | None = None, info: torch.Tensor | None = None,
+                                queue: Queue | None = None):
+    import numpy as np
+    lib = _ffi.load()
+    queue = queue or default_queue(A.data.device)
+    ipiv = ipiv or IpivV(np.minimum(A.m, A.n), A.data.device)
+    info = info if info is not None else torch.zeros(len(A), dtype=torch.int32, device=A.data.device)
+    lwork = ctypes.c_int(-1)
+    max_m, max_n = int(A.m.max()), int(A.n.max())
+    mn = int(np.minimum(A.m, A.n).max())
+    mxn = int((A.m.astype(np.int64) * A.n).max())
+    lib.magma_dgetrf_vbatched_max_nocheck_work(None, None, max_m, max_n, mn, mxn, None, None, None, None, None,
+                                               ctypes.byref(lwork), len(A), queue.handle)
+    work = torch.empty(max(lwork.value, 1), dtype=torch.uint8, device=A.data.device)
+    rc = lib.magma_dgetrf_vbatched_max_nocheck_work(A.d_m.data_ptr(), A.d_n.data_ptr(), max_m, max_n, mn, mxn,
+                                                    A.ptrs.data_ptr(), A.d_ld.data_ptr(), ipiv.ptrs.data_ptr(),
+                                                    info.data_ptr(), work.data_ptr(), ctypes.byref(lwork),
+                                                    len(A), queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgetrf_vbatched_max_nocheck_work", rc)
+    return ipiv, info
+
+
+def dgetrs_vbatched(trans: int, A: VBatch, ipiv: IpivV, B: VBatch, queue: Queue | None = None) -> None:
+    queue = queue or default_queue(A.data.device)
+    rc = _ffi.load().magma_dgetrs_vbatched(trans, A.d_n.data_ptr(), B.d_n.data_ptr(), int(A.n.max()),
+                                           int(B.n.max()), A.ptrs.data_ptr(), A.d_ld.data_ptr(),
+                                           ipiv.ptrs.data_ptr(), B.ptrs.data_ptr(), B.d_ld.data_ptr(), len(A),
+                                           queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgetrs_vbatched", rc)
+
+
+def dgesv_vbatched(A: VBatch, B: VBatch, ipiv: IpivV | None = None, info: torch.Tensor | None = None,
+                   queue: Queue | None = None):
+    """B is a VBatch with m = n_i rows and n = nrhs_i columns."""
+    queue = queue or default_queue(A.data.device)
+    ipiv = ipiv or IpivV(A.n, A.data.device)
+    info = info if info is not None else torch.zeros(len(A), dtype=torch.int32, device=A.data.device)
+    rc = _ffi.load().magma_dgesv_vbatched(A.d_n.data_ptr(), B.d_n.data_ptr(), int(A.n.max()), int(B.n.max()),
+                                          A.ptrs.data_ptr(), A.d_ld.data_ptr(), ipiv.ptrs.data_ptr(),
+                                          B.ptrs.data_ptr(), B.d_ld.data_ptr(), info.data_ptr(), len(A),
+                                          queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgesv_vbatched", rc)
+    return ipiv, info
+
+
+class VbatchedPlan:
+    """Host-side bucketing of a variable-size solve, executed or replayed as a
+    hipGraph without allocation or synchronisation (magma_dgesv_vbatched_plan_*)."""
+
+    def __init__(self, n, nrhs, variants=None, queue: Queue | None = None):
+        import numpy as np
+        self.queue = queue or default_queue()
+        self._lib = _ffi.load()
+        self._n = np.ascontiguousarray(n, dtype=np.int32)
+        self._r = None if nrhs is None else np.ascontiguousarray(nrhs, dtype=np.int32)
+        v = None
+        if variants is not None:
+            v = np.zeros(22, dtype=np.int32)
+            for k, val in dict(variants).items():
+                v[int(k)] = int(val)
+        self._v = v
+        h = ctypes.c_void_p()
+        rc = self._lib.magma_dgesv_vbatched_plan_create(
+            self._n.ctypes.data, None if self._r is None else self._r.ctypes.data, len(self._n),
+            None if v is None else v.ctypes.data, self.queue.handle, ctypes.byref(h))
+        if rc < 0:
+            raise MagmaError("magma_dgesv_vbatched_plan_create", rc)
+        self.handle = h
+
+    def buckets(self):
+        out = []
+        info = (ctypes.c_int * 5)()
+        for i in range(self._lib.magma_vbatched_plan_num_buckets(self.handle)):
+            self._lib.magma_vbatched_plan_bucket(self.handle, i, info)
+            out.append(dict(id=info[0], variant=info[1], count=info[2], max_n=info[3], max_nrhs=info[4]))
+        return out
+
+    def _args(self, A: VBatch, B: VBatch, ipiv: IpivV, info: torch.Tensor):
+        return (A.d_n.data_ptr(), B.d_n.data_ptr(), A.ptrs.data_ptr(), A.d_ld.data_ptr(), ipiv.ptrs.data_ptr(),
+                B.ptrs.data_ptr(), B.d_ld.data_ptr(), info.data_ptr())
+
+    def execute(self, A, B, ipiv, info):
+        rc = self._lib.magma_dgesv_vbatched_plan_exec(self.handle, *self._args(A, B, ipiv, info), self.queue.handle)
+        if rc < 0:
+            raise MagmaError("magma_dgesv_vbatched_plan_exec", rc)
+
+    def capture(self, A, B, ipiv, info):
+        rc = self._lib.magma_dgesv_vbatched_plan_capture(self.handle, *self._args(A, B, ipiv, info),
+                                                         self.queue.handle)
+        if rc < 0:
+            raise MagmaError("magma_dgesv_vbatched_plan_capture", rc)
+
+    def replay(self):
+        rc = self._lib.magma_vbatched_plan_replay(self.handle, self.queue.handle)
+        if rc < 0:
+            raise MagmaError("magma_vbatched_plan_replay", rc)
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.magma_vbatched_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

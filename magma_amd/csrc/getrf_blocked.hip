@@ -1,19 +1,25 @@
 // Blocked right-looking batched LU for matrices too large for one workgroup's
-// on-chip storage, fixed or variable size. Per block column of width jb:
+// on-chip storage, fixed or variable size, FP64 or FP32 storage. Per block
+// column of width jb:
 //   1. panel:  LU with partial pivoting of the (m-k) x jb panel, rows spread
 //              over the workgroup's threads in registers;
 //   2. laswp:  the panel's interchanges applied to the columns left and right
 //              of the panel;
 //   3. trsm:   A12 := L11^{-1} A12 (unit lower);
-//   4. gemm:   A22 := A22 - A21 * A12, FP64 MFMA tiles.
+//   4. gemm:   A22 := A22 - A21 * A12 on MFMA. FP64 storage uses
+//              v_mfma_f64_16x16x4f64. FP32 storage uses, by update format:
+//              FP32 inputs (v_mfma_f32_16x16x4f32), BF16 or FP16 inputs with
+//              FP32 accumulation (v_mfma_f32_16x16x32_{bf16,f16}), or a
+//              three-term BF16 split (hi*hi + hi*lo + lo*hi).
 // Matrices whose size is exhausted exit at once, so one launch per step
 // serves a batch of mixed sizes.
 //
 // Reference structure: src/zgetrf_batched.cpp:149-203 (blocked loop),
 // src/zgetrf_panel_batched.cpp:102-196 (panel, laswp, trsm, gemm per step),
-// src/zgetrf_vbatched.cpp:20-124 (variable sizes, per-kernel early exit).
-// Pivot, scaling and zero-pivot semantics follow LAPACK dgetf2 as in
-// getrf_lds.hip.
+// src/zgetrf_vbatched.cpp:20-124 (variable sizes, per-kernel early exit),
+// src/xshgetrf_gpu.cpp:214-350 (FP32 panel, FP16-input / FP32-accumulate
+// trailing update). Pivot, scaling and zero-pivot semantics follow LAPACK
+// dgetf2 as in getrf_lds.hip.
 #include "magma_internal.h"
 #include "lu_device.h"
 #include "getrf_kernels.h"
@@ -22,33 +28,28 @@ namespace mgi {
 
 constexpr int kPanelThreads = 256;
 
-struct SizeArgs {
-    int m, n, ld;                 // fixed-size values
-    const int* m_arr;             // per-matrix sizes (nullptr for fixed size)
-    const int* n_arr;
-    const int* ld_arr;
-    __device__ __forceinline__ void get(int b, int& mm, int& nn, int& ll) const
-    {
-        if (m_arr) { mm = m_arr[b]; nn = n_arr[b]; ll = ld_arr[b]; }
-        else { mm = m; nn = n; ll = ld; }
-    }
-};
+template <typename T> struct Limits;
+template <> struct Limits<double> { static constexpr double sfmin = kSfminD; };
+template <> struct Limits<float>  { static constexpr float sfmin = kSfminF; };
+
+__device__ __forceinline__ void wave_argmax_t(double& v, int& idx) { wave_argmax(v, idx); }
+__device__ __forceinline__ void wave_argmax_t(float& v, int& idx) { wave_argmax_f(v, idx); }
 
 // ---------------------------------------------------------------------------
 // 1. Panel: thread t owns panel rows t + 256*i (i < R); the jb <= NB columns
 //    of those rows sit in registers.
 // ---------------------------------------------------------------------------
-template <int R, int NB>
+template <typename T, int R, int NB>
 __global__ void __launch_bounds__(kPanelThreads)
-panel_kernel(SizeArgs sz, int k, double** __restrict__ dA_array, int** __restrict__ ipiv_array,
+panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
              int* __restrict__ info_array)
 {
-    __shared__ double s_val[kPanelThreads / kWave];
+    __shared__ T s_val[kPanelThreads / kWave];
     __shared__ int s_idx[kPanelThreads / kWave];
-    __shared__ double s_rowj[NB];
-    __shared__ double s_rowp[NB];
+    __shared__ T s_rowj[NB];
+    __shared__ T s_rowp[NB];
 
-    const int b = blockIdx.x;
+    const int b = sz.mat(blockIdx.x);
     int m, n, ld;
     sz.get(b, m, n, ld);
     const int minmn = min(m, n);
@@ -56,101 +57,95 @@ panel_kernel(SizeArgs sz, int k, double** __restrict__ dA_array, int** __restric
     const int jb = min(NB, minmn - k);
     const int mloc = m - k;
     const int t = threadIdx.x;
-    double* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
+    T* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
     int* ipiv = ipiv_array[b];
 
-    double a[R][NB];
+    T a[R][NB];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int r = t + kPanelThreads * i;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-            a[i][j] = (r < mloc && j < jb) ? A[r + (size_t)j * ld] : 0.0;
+            a[i][j] = (r < mloc && j < jb) ? A[r + (size_t)j * ld] : T(0);
     }
 
     int info = 0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        if (j >= jb) break;
-        // arg-max of |a(r, j)| over rows r >= j
-        double best = -1.0;
-        int bi = 0x7fffffff;
+        // constant trip count (cross-lane operations inside)
+        if (j < jb) {
+            T best = T(-1);
+            int bi = 0x7fffffff;
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-            const int r = t + kPanelThreads * i;
-            if (r >= j && r < mloc) {
-                double v = fabs(a[i][j]);
-                if (v > best) { best = v; bi = r; }
-            }
-        }
-        wave_argmax(best, bi);
-        const int w = t / kWave;
-        if ((t & (kWave - 1)) == 0) { s_val[w] = best; s_idx[w] = bi; }
-        __syncthreads();
-        best = s_val[0];
-        bi = s_idx[0];
-#pragma unroll
-        for (int q = 1; q < kPanelThreads / kWave; ++q) {
-            double ov = s_val[q];
-            int oi = s_idx[q];
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        const int p = bi;
-        // Row j (thread j, slot 0) and row p (thread p % 256, slot p / 256)
-        // publish their panel rows.
-        if (t == j) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) s_rowj[c] = a[0][c];
-        }
-#pragma unroll
-        for (int i = 0; i < R; ++i)
-            if (t + kPanelThreads * i == p) {
-#pragma unroll
-                for (int c = 0; c < NB; ++c) s_rowp[c] = a[i][c];
-            }
-        __syncthreads();
-        const double pv = s_rowp[j];
-        if (t == 0) ipiv[k + j] = k + p + 1;
-        if (pv != 0.0) {
-            if (p != j) {
-                if (t == j) {
-#pragma unroll
-                    for (int c = 0; c < NB; ++c) a[0][c] = s_rowp[c];
+            for (int i = 0; i < R; ++i) {
+                const int r = t + kPanelThreads * i;
+                if (r >= j && r < mloc) {
+                    T v = fabs(a[i][j]);
+                    if (v > best) { best = v; bi = r; }
                 }
-#pragma unroll
-                for (int i = 0; i < R; ++i)
-                    if (t + kPanelThreads * i == p) {
-#pragma unroll
-                        for (int c = 0; c < NB; ++c) a[i][c] = s_rowj[c];
-                    }
             }
-            const bool recip = fabs(pv) >= kSfminD;
-            const double rinv = 1.0 / pv;
+            wave_argmax_t(best, bi);
+            const int w = t / kWave;
+            if ((t & (kWave - 1)) == 0) { s_val[w] = best; s_idx[w] = bi; }
+            __syncthreads();
+            best = s_val[0];
+            bi = s_idx[0];
+#pragma unroll
+            for (int q = 1; q < kPanelThreads / kWave; ++q) {
+                T ov = s_val[q];
+                int oi = s_idx[q];
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            const int p = bi;
+            // Row j (thread j, slot 0) and row p (thread p % 256, slot p / 256)
+            // publish their panel rows.
+            if (t == j) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) s_rowj[c] = a[0][c];
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (t + kPanelThreads * i == p) {
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) s_rowp[c] = a[i][c];
+                }
+            __syncthreads();
+            const T pv = s_rowp[j];
+            if (t == 0) ipiv[k + j] = k + p + 1;
+            T rinv = T(1);
+            bool recip = true;
+            if (pv != T(0)) {
+                if (p != j) {
+                    if (t == j) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) a[0][c] = s_rowp[c];
+                    }
+#pragma unroll
+                    for (int i = 0; i < R; ++i)
+                        if (t + kPanelThreads * i == p) {
+#pragma unroll
+                            for (int c = 0; c < NB; ++c) a[i][c] = s_rowj[c];
+                        }
+                }
+                recip = fabs(pv) >= Limits<T>::sfmin;
+                rinv = T(1) / pv;
+            } else if (info == 0) {
+                info = k + j + 1;
+            }
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 const int r = t + kPanelThreads * i;
                 if (r > j && r < mloc) {
-                    const double l = recip ? a[i][j] * rinv : a[i][j] / pv;
+                    T l = a[i][j];
+                    if (pv != T(0)) l = recip ? l * rinv : l / pv;
                     a[i][j] = l;
 #pragma unroll
                     for (int c = 0; c < NB; ++c)
                         if (c > j && c < jb) a[i][c] = fma(-l, s_rowp[c], a[i][c]);
                 }
             }
-        } else {
-            if (info == 0) info = k + j + 1;
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                const int r = t + kPanelThreads * i;
-                if (r > j && r < mloc) {
-                    const double l = a[i][j];
-#pragma unroll
-                    for (int c = 0; c < NB; ++c)
-                        if (c > j && c < jb) a[i][c] = fma(-l, s_rowp[c], a[i][c]);
-                }
-            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 
 #pragma unroll
@@ -168,10 +163,11 @@ panel_kernel(SizeArgs sz, int k, double** __restrict__ dA_array, int** __restric
 // ---------------------------------------------------------------------------
 // 2. Row interchanges of panel k..k+jb-1 on every column outside the panel.
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ void __launch_bounds__(256)
-laswp_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array, int** __restrict__ ipiv_array)
+laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __restrict__ ipiv_array)
 {
-    const int b = blockIdx.y;
+    const int b = sz.mat(blockIdx.y);
     int m, n, ld;
     sz.get(b, m, n, ld);
     const int minmn = min(m, n);
@@ -181,14 +177,14 @@ laswp_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array, int** _
     const int other = n - jb;                       // columns outside the panel
     if (c >= other) return;
     const int col = c < k ? c : c + jb;
-    double* A = dA_array[b] + (size_t)col * ld;
+    T* A = dA_array[b] + (size_t)col * ld;
     const int* ipiv = ipiv_array[b];
     for (int j = k; j < k + jb; ++j) {
         const int p = ipiv[j] - 1;
-        if (p != j && p > j && p < m) {
-            double t = A[j];
+        if (p > j && p < m) {
+            T tmp = A[j];
             A[j] = A[p];
-            A[p] = t;
+            A[p] = tmp;
         }
     }
 }
@@ -196,12 +192,12 @@ laswp_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array, int** _
 // ---------------------------------------------------------------------------
 // 3. A12 := L11^{-1} A12, one thread per column of A12.
 // ---------------------------------------------------------------------------
-template <int NB>
+template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-trsm_kernel(SizeArgs sz, int k, double** __restrict__ dA_array)
+trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array)
 {
-    __shared__ double sL[NB * NB];
-    const int b = blockIdx.y;
+    __shared__ T sL[NB * NB];
+    const int b = sz.mat(blockIdx.y);
     int m, n, ld;
     sz.get(b, m, n, ld);
     const int minmn = min(m, n);
@@ -211,7 +207,7 @@ trsm_kernel(SizeArgs sz, int k, double** __restrict__ dA_array)
     if (ncols <= 0) return;
     const int c0 = blockIdx.x * blockDim.x;
     if (c0 >= ncols) return;
-    const double* L = dA_array[b] + k + (size_t)k * ld;
+    const T* L = dA_array[b] + k + (size_t)k * ld;
     for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
         const int i = idx % jb, j = idx / jb;
         sL[i + j * NB] = L[i + (size_t)j * ld];
@@ -219,10 +215,10 @@ trsm_kernel(SizeArgs sz, int k, double** __restrict__ dA_array)
     __syncthreads();
     const int c = c0 + threadIdx.x;
     if (c >= ncols) return;
-    double* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
-    double x[NB];
+    T* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
+    T x[NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) x[i] = i < jb ? X[i] : 0.0;
+    for (int i = 0; i < NB; ++i) x[i] = i < jb ? X[i] : T(0);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
 #pragma unroll
@@ -235,24 +231,31 @@ trsm_kernel(SizeArgs sz, int k, double** __restrict__ dA_array)
 }
 
 // ---------------------------------------------------------------------------
-// 4. A22 := A22 - A21 * A12 with v_mfma_f64_16x16x4f64. A workgroup of 4
-//    waves computes a 64x64 tile of A22 (each wave a 32x32 quarter = 2x2
-//    MFMA tiles); K = jb <= NB is staged in LDS.
+// 4. A22 := A22 - A21 * A12. A workgroup of 4 waves computes a 64x64 tile of
+//    A22 (each wave a 32x32 quarter = 2x2 MFMA tiles); K = jb <= NB is staged
+//    in LDS.
 //
-//    MFMA f64 16x16x4 operand layout (lane l, 64 lanes):
-//      A (16x4):  A[l % 16][l / 16]
-//      B (4x16):  B[l / 16][l % 16]
-//      C (16x16): 4 values per lane, C[4*r + l/16][l % 16], r = 0..3
+//    MFMA operand layouts (lane l of 64):
+//      16x16x4 (f64, f32):  A[l%16][l/16], B[l/16][l%16]
+//      16x16x32 (bf16, f16): A[l%16][8*(l/16)+v], B[8*(l/16)+v][l%16], v < 8
+//    accumulators (4 per lane, r = 0..3):
+//      f64:  C[4*r + l/16][l%16]
+//      f32:  C[4*(l/16) + r][l%16]
 // ---------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 
-template <int NB>
+template <typename T, int NB, int MODE>
 __global__ void __launch_bounds__(256)
-gemm_update_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array)
+gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
 {
-    __shared__ double sA[NB][64 + 1];     // A21 tile, [kk][row]
-    __shared__ double sB[NB][64 + 1];     // A12 tile, [kk][col]
-    const int b = blockIdx.z;
+    constexpr bool kWide = (MODE == kUpdBF16 || MODE == kUpdFP16 || MODE == kUpdBF16x3);
+    constexpr int KP = kWide ? 32 : NB;              // K staged (zero padded)
+    __shared__ T sA[KP][64 + 1];                     // A21 tile, [kk][row]
+    __shared__ T sB[KP][64 + 1];                     // A12 tile, [kk][col]
+    const int b = sz.mat(blockIdx.z);
     int m, n, ld;
     sz.get(b, m, n, ld);
     const int minmn = min(m, n);
@@ -261,61 +264,151 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, double** __restrict__ dA_array)
     const int mr = m - k - jb, nc = n - k - jb;
     const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
     if (r0 >= mr || c0 >= nc) return;
-    double* A = dA_array[b];
-    const double* A21 = A + (k + jb) + (size_t)k * ld;
-    const double* A12 = A + k + (size_t)(k + jb) * ld;
-    double* C = A + (k + jb) + (size_t)(k + jb) * ld;
+    T* A = dA_array[b];
+    const T* A21 = A + (k + jb) + (size_t)k * ld;
+    const T* A12 = A + k + (size_t)(k + jb) * ld;
+    T* C = A + (k + jb) + (size_t)(k + jb) * ld;
     const int t = threadIdx.x;
-    for (int idx = t; idx < NB * 64; idx += 256) {
+    for (int idx = t; idx < KP * 64; idx += 256) {
         const int r = idx % 64, kk = idx / 64;           // A21: walk down columns
-        sA[kk][r] = (kk < jb && r0 + r < mr) ? A21[r0 + r + (size_t)kk * ld] : 0.0;
-        const int kb = idx % NB, cc = idx / NB;          // A12: walk down its short columns
-        sB[kb][cc] = (kb < jb && c0 + cc < nc) ? A12[kb + (size_t)(c0 + cc) * ld] : 0.0;
+        sA[kk][r] = (kk < jb && r0 + r < mr) ? A21[r0 + r + (size_t)kk * ld] : T(0);
+        const int kb = idx % KP, cc = idx / KP;          // A12: walk down its short columns
+        sB[kb][cc] = (kb < jb && c0 + cc < nc) ? A12[kb + (size_t)(c0 + cc) * ld] : T(0);
     }
     __syncthreads();
     const int w = t / kWave, l = t % kWave;
     const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
-    double4_t acc[2][2];
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-            acc[ti][tj] = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int kb = 0; kb < NB; kb += 4) {
-        if (kb >= jb) break;
-        double af[2], bf[2];
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti) af[ti] = sA[kb + l / 16][wr + 16 * ti + l % 16];
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj) bf[tj] = sB[kb + l / 16][wc + 16 * tj + l % 16];
+    const int lr = l % 16, lq = l / 16;
+
+    if constexpr (MODE == kUpdF64) {
+        double4_t acc[2][2];
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-            for (int tj = 0; tj < 2; ++tj)
-                acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
-    }
+            for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = double4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+        for (int kb = 0; kb < NB; kb += 4) {
+            if (kb < jb) {
+                double af[2], bf[2];
 #pragma unroll
-        for (int tj = 0; tj < 2; ++tj) {
-            const int col = c0 + wc + 16 * tj + l % 16;
-            if (col >= nc) continue;
+                for (int ti = 0; ti < 2; ++ti) af[ti] = sA[kb + lq][wr + 16 * ti + lr];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + wr + 16 * ti + 4 * r + l / 16;
-                if (row < mr) {
-                    double* cp = C + row + (size_t)col * ld;
-                    *cp = *cp - acc[ti][tj][r];
-                }
+                for (int tj = 0; tj < 2; ++tj) bf[tj] = sB[kb + lq][wc + 16 * tj + lr];
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj)
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const int col = c0 + wc + 16 * tj + lr;
+                if (col >= nc) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = r0 + wr + 16 * ti + 4 * r + lq;
+                    if (row < mr) {
+                        T* cp = C + row + (size_t)col * ld;
+                        *cp = *cp - acc[ti][tj][r];
+                    }
+                }
+            }
+    } else {
+        float4_t acc[2][2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = float4_t{0.f, 0.f, 0.f, 0.f};
+        if constexpr (MODE == kUpdF32) {
+#pragma unroll
+            for (int kb = 0; kb < NB; kb += 4) {
+                if (kb < jb) {
+                    float af[2], bf[2];
+#pragma unroll
+                    for (int ti = 0; ti < 2; ++ti) af[ti] = sA[kb + lq][wr + 16 * ti + lr];
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) bf[tj] = sB[kb + lq][wc + 16 * tj + lr];
+#pragma unroll
+                    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < 2; ++tj)
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+                }
+            }
+        } else {
+            // one K=32 step: lane holds k = 8*(l/16) + v, v < 8
+            float av[2][8], bv[2][8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti) av[ti][v] = sA[8 * lq + v][wr + 16 * ti + lr];
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) bv[tj][v] = sB[8 * lq + v][wc + 16 * tj + lr];
+            }
+            if constexpr (MODE == kUpdFP16) {
+                f16x8_t ah[2], bh[2];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        ah[q][v] = (_Float16)av[q][v];
+                        bh[q][v] = (_Float16)bv[q][v];
+                    }
+                }
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj)
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+            } else {
+                bf16x8_t ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        ah[q][v] = (__bf16)av[q][v];
+                        bh[q][v] = (__bf16)bv[q][v];
+                        al[q][v] = (__bf16)(av[q][v] - (float)ah[q][v]);
+                        bl[q][v] = (__bf16)(bv[q][v] - (float)bh[q][v]);
+                    }
+                }
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) {
+                        if constexpr (MODE == kUpdBF16x3) {
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bl[tj], acc[ti][tj], 0, 0, 0);
+                        }
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                    }
+            }
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const int col = c0 + wc + 16 * tj + lr;
+                if (col >= nc) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = r0 + wr + 16 * ti + 4 * lq + r;
+                    if (row < mr) {
+                        T* cp = C + row + (size_t)col * ld;
+                        *cp = *cp - acc[ti][tj][r];
+                    }
+                }
+            }
+    }
 }
 
-__global__ void fill_int_kernel(int* p, int v, int count)
+__global__ void fill_int_kernel(int* p, int v, int count, const int* map)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
+    if (i < count) p[map ? map[i] : i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -336,31 +429,31 @@ static bool plan_for(int max_m, BlockedPlan& p)
     return true;
 }
 
-template <int NB>
+template <typename T, int NB, int MODE>
 static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n, int max_minmn,
-                                 double** dA_array, int** ipiv_array, int* info_array, int batch,
+                                 T** dA_array, int** ipiv_array, int* info_array, int batch,
                                  hipStream_t stream)
 {
     for (int k = 0; k < max_minmn; k += NB) {
         switch (R) {
-            case 1: panel_kernel<1, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 2: panel_kernel<2, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 4: panel_kernel<4, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 8: panel_kernel<8, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 1: panel_kernel<T, 1, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 2: panel_kernel<T, 2, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 4: panel_kernel<T, 4, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
+            case 8: panel_kernel<T, 8, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
             default: return MAGMA_ERR_NOT_SUPPORTED;
         }
         MGI_LAUNCH_CHECK();
         const int other = max_n;   // upper bound of the columns outside the panel
-        laswp_kernel<<<dim3(ceildiv(other, 256), batch), 256, 0, stream>>>(sz, k, NB, dA_array, ipiv_array);
+        laswp_kernel<T><<<dim3(ceildiv(other, 256), batch), 256, 0, stream>>>(sz, k, NB, dA_array, ipiv_array);
         MGI_LAUNCH_CHECK();
         const int rest_n = max_n - k - 1;
         const int rest_m = max_m - k - 1;
         if (rest_n > 0) {
-            trsm_kernel<NB><<<dim3(ceildiv(rest_n, 256), batch), 256, 0, stream>>>(sz, k, dA_array);
+            trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, 256), batch), 256, 0, stream>>>(sz, k, dA_array);
             MGI_LAUNCH_CHECK();
             if (rest_m > 0) {
                 dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
-                gemm_update_kernel<NB><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array);
+                gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array);
                 MGI_LAUNCH_CHECK();
             }
         }
@@ -370,20 +463,61 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
 
 bool getrf_blocked_supported(int max_m) { BlockedPlan p; return plan_for(max_m, p); }
 
+// Panel widths a given row count supports: the panel keeps R x NB values per
+// thread in registers, R = ceil(max_m / 256), and R * NB <= 64.
+bool blocked_nb_valid(int max_m, int nb)
+{
+    BlockedPlan p;
+    if (!plan_for(max_m, p)) return false;
+    return (nb == 8 || nb == 16 || nb == 32) && p.rows_per_thread * nb <= 64;
+}
+
+template <typename T, int MODE>
+static magma_int_t blocked_launch(const SizeArgs& sz, int max_m, int max_n, T** dA_array, int** ipiv_array,
+                                  int* info_array, int batch, hipStream_t stream, int force_nb = 0)
+{
+    BlockedPlan plan;
+    if (!plan_for(max_m, plan)) return MAGMA_ERR_NOT_SUPPORTED;
+    if (force_nb != 0) {
+        if (!blocked_nb_valid(max_m, force_nb)) return MAGMA_ERR_NOT_SUPPORTED;
+        plan.nb = force_nb;
+    }
+    fill_int_kernel<<<ceildiv(batch, 256), 256, 0, stream>>>(info_array, 0, batch, sz.map);
+    MGI_LAUNCH_CHECK();
+    const int max_minmn = std::min(max_m, max_n);
+    switch (plan.nb) {
+        case 32: return blocked_steps<T, 32, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        case 16: return blocked_steps<T, 16, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        case 8:  return blocked_steps<T, 8, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+        default: return MAGMA_ERR_NOT_SUPPORTED;
+    }
+}
+
 magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const int* n_arr, const int* ld_arr,
                                  int max_m, int max_n, double** dA_array, int** ipiv_array, int* info_array,
                                  int batch, hipStream_t stream)
 {
-    BlockedPlan plan;
-    if (!plan_for(max_m, plan)) return MAGMA_ERR_NOT_SUPPORTED;
-    fill_int_kernel<<<ceildiv(batch, 256), 256, 0, stream>>>(info_array, 0, batch);
-    MGI_LAUNCH_CHECK();
-    SizeArgs sz{m, n, ldda, m_arr, n_arr, ld_arr};
-    const int max_minmn = std::min(max_m, max_n);
-    switch (plan.nb) {
-        case 32: return blocked_steps<32>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        case 16: return blocked_steps<16>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        case 8:  return blocked_steps<8>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
+    SizeArgs sz{m, n, ldda, m_arr, n_arr, ld_arr, nullptr};
+    return blocked_launch<double, kUpdF64>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream);
+}
+
+magma_int_t getrf_blocked_launch_sz(const SizeArgs& sz, int max_m, int max_n, double** dA_array, int** ipiv_array,
+                                    int* info_array, int batch, hipStream_t stream, int force_nb)
+{
+    return blocked_launch<double, kUpdF64>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream,
+                                           force_nb);
+}
+
+magma_int_t sgetrf_blocked_launch(int mode, int m, int n, int ldda, const int* m_arr, const int* n_arr,
+                                  const int* ld_arr, int max_m, int max_n, float** dA_array, int** ipiv_array,
+                                  int* info_array, int batch, hipStream_t stream)
+{
+    SizeArgs sz{m, n, ldda, m_arr, n_arr, ld_arr, nullptr};
+    switch (mode) {
+        case kUpdF32:   return blocked_launch<float, kUpdF32>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream);
+        case kUpdBF16:  return blocked_launch<float, kUpdBF16>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream);
+        case kUpdFP16:  return blocked_launch<float, kUpdFP16>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream);
+        case kUpdBF16x3: return blocked_launch<float, kUpdBF16x3>(sz, max_m, max_n, dA_array, ipiv_array, info_array, batch, stream);
         default: return MAGMA_ERR_NOT_SUPPORTED;
     }
 }

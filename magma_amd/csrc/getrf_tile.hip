@@ -83,8 +83,8 @@ struct TileSmem {
 
 template <int NT, bool WITH_B>
 __global__ void __launch_bounds__(kTileThreads, 2)
-getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int** __restrict__ ipiv_array,
-                  int* __restrict__ info_array, double** __restrict__ dB_array, int lddb)
+getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict__ ipiv_array,
+                  int* __restrict__ info_array, double** __restrict__ dB_array)
 {
     constexpr int NP = 16 * NT;
     constexpr int NTW = (NT + 1) / 2;
@@ -100,8 +100,14 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
     const int wr = w >> 1, wc = w & 1;
     const int lr = l % 16;            // row within a tile
     const int lq = l / 16;            // column group within a tile
-    double* A = dA_array[blockIdx.x];
-    int* ipiv = ipiv_array[blockIdx.x];
+    const int mat = ta.map ? ta.map[blockIdx.x] : blockIdx.x;
+    const int n = ta.n_arr ? ta.n_arr[mat] : ta.n;
+    const int nrhs = ta.nrhs_arr ? ta.nrhs_arr[mat] : ta.nrhs;
+    const int ldda = ta.ldda_arr ? ta.ldda_arr[mat] : ta.ldda;
+    const int lddb = ta.lddb_arr ? ta.lddb_arr[mat] : ta.lddb;
+    if (n <= 0) return;                   // empty system: nothing to factor
+    double* A = dA_array[mat];
+    int* ipiv = ipiv_array[mat];
 
     // ---- load tiles (padding = identity) ----
     double4_t acc[NTW][NTW];
@@ -123,7 +129,7 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
         }
     }
     if constexpr (WITH_B) {
-        const double* B = dB_array[blockIdx.x];
+        const double* B = dB_array[mat];
         for (int idx = t; idx < NP * 16; idx += kTileThreads) {
             const int c = idx / NP, row = idx % NP;
             sm.B[row * LD16 + c] = (row < n && c < nrhs) ? B[row + (size_t)c * lddb] : 0.0;
@@ -484,7 +490,7 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
             }
         }
     }
-    if (t == 0) info_array[blockIdx.x] = sm.info;
+    if (t == 0) info_array[mat] = sm.info;
 
     if constexpr (WITH_B) {
         // ---- back substitution U X = Y, block rows bottom-up ----
@@ -555,7 +561,7 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
             }
             __syncthreads();
         }
-        double* B = dB_array[blockIdx.x];
+        double* B = dB_array[mat];
         for (int idx = t; idx < n * nrhs; idx += kTileThreads) {
             const int row = idx % n, c = idx / n;
             B[row + (size_t)c * lddb] = sm.B[row * LD16 + c];
@@ -563,18 +569,15 @@ getrf_tile_kernel(int n, int nrhs, double** __restrict__ dA_array, int ldda, int
     }
 }
 
-bool getrf_tile_supported(int n) { return n >= 1 && n <= 128; }
-
 template <bool WITH_B>
-static magma_int_t launch_nt(int n, int nrhs, double** dA_array, int ldda, int** ipiv_array, int* info_array,
-                             double** dB_array, int lddb, int batch, hipStream_t stream)
+static magma_int_t launch_nt(int nt, const TileArgs& a, double** dA_array, int** ipiv_array, int* info_array,
+                             double** dB_array, int batch, hipStream_t stream)
 {
-    const int nt = (n + 15) / 16;
     dim3 grid(batch), block(kTileThreads);
-#define MGI_TILE_CASE(NTV)                                                                              \
-    case NTV:                                                                                           \
-        getrf_tile_kernel<NTV, WITH_B><<<grid, block, 0, stream>>>(n, nrhs, dA_array, ldda, ipiv_array, \
-                                                                   info_array, dB_array, lddb);         \
+#define MGI_TILE_CASE(NTV)                                                                               \
+    case NTV:                                                                                            \
+        getrf_tile_kernel<NTV, WITH_B><<<grid, block, 0, stream>>>(a, dA_array, ipiv_array, info_array, \
+                                                                   dB_array);                           \
         break;
     switch (nt) {
         MGI_TILE_CASE(1)
@@ -592,11 +595,26 @@ static magma_int_t launch_nt(int n, int nrhs, double** dA_array, int ldda, int**
     return MAGMA_SUCCESS;
 }
 
+bool getrf_tile_supported(int n) { return n >= 1 && n <= 128; }
+
+magma_int_t getrf_tile_launch_args(int nt, const TileArgs& a, double** dA_array, int** ipiv_array, int* info_array,
+                                   int batch, hipStream_t stream)
+{
+    return launch_nt<false>(nt, a, dA_array, ipiv_array, info_array, nullptr, batch, stream);
+}
+
+magma_int_t gesv_tile_launch_args(int nt, const TileArgs& a, double** dA_array, int** ipiv_array, int* info_array,
+                                  double** dB_array, int batch, hipStream_t stream)
+{
+    return launch_nt<true>(nt, a, dA_array, ipiv_array, info_array, dB_array, batch, stream);
+}
+
 magma_int_t getrf_tile_launch(int n, double** dA_array, int ldda, int** ipiv_array, int* info_array,
                               int batch, hipStream_t stream)
 {
     if (!getrf_tile_supported(n)) return MAGMA_ERR_NOT_SUPPORTED;
-    return launch_nt<false>(n, 0, dA_array, ldda, ipiv_array, info_array, nullptr, 0, batch, stream);
+    TileArgs a{n, 0, ldda, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return launch_nt<false>((n + 15) / 16, a, dA_array, ipiv_array, info_array, nullptr, batch, stream);
 }
 
 bool gesv_tile_supported(int n, int nrhs) { return getrf_tile_supported(n) && nrhs >= 1 && nrhs <= 16; }
@@ -605,7 +623,8 @@ magma_int_t gesv_tile_launch(int n, int nrhs, double** dA_array, int ldda, int**
                              double** dB_array, int lddb, int batch, hipStream_t stream)
 {
     if (!gesv_tile_supported(n, nrhs)) return MAGMA_ERR_NOT_SUPPORTED;
-    return launch_nt<true>(n, nrhs, dA_array, ldda, ipiv_array, info_array, dB_array, lddb, batch, stream);
+    TileArgs a{n, nrhs, ldda, lddb, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return launch_nt<true>((n + 15) / 16, a, dA_array, ipiv_array, info_array, dB_array, batch, stream);
 }
 
 }  // namespace mgi

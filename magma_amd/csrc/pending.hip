@@ -5,34 +5,6 @@
 
 extern "C" {
 
-magma_int_t magma_dgetrf_vbatched(magma_int_t*, magma_int_t*, double**, magma_int_t*, magma_int_t**,
-                                  magma_int_t*, magma_int_t, magma_queue_t)
-{
-    return MAGMA_ERR_NOT_IMPLEMENTED;
-}
-
-magma_int_t magma_dgetrf_vbatched_max_nocheck_work(magma_int_t*, magma_int_t*, magma_int_t, magma_int_t,
-                                                   magma_int_t, magma_int_t, double**, magma_int_t*,
-                                                   magma_int_t**, magma_int_t*, void*, magma_int_t*,
-                                                   magma_int_t, magma_queue_t)
-{
-    return MAGMA_ERR_NOT_IMPLEMENTED;
-}
-
-magma_int_t magma_dgetrs_vbatched(magma_trans_t, magma_int_t*, magma_int_t*, magma_int_t, magma_int_t,
-                                  double**, magma_int_t*, magma_int_t**, double**, magma_int_t*, magma_int_t,
-                                  magma_queue_t)
-{
-    return MAGMA_ERR_NOT_IMPLEMENTED;
-}
-
-magma_int_t magma_dgesv_vbatched(magma_int_t*, magma_int_t*, magma_int_t, magma_int_t, double**, magma_int_t*,
-                                 magma_int_t**, double**, magma_int_t*, magma_int_t*, magma_int_t,
-                                 magma_queue_t)
-{
-    return MAGMA_ERR_NOT_IMPLEMENTED;
-}
-
 magma_int_t magma_dhgesv_iteref_gpu(magma_trans_t, magma_int_t, magma_int_t, magmaDouble_ptr, magma_int_t,
                                     magma_int_t*, magmaInt_ptr, magmaDouble_ptr, magma_int_t, magmaDouble_ptr,
                                     magma_int_t, magmaDouble_ptr, magmaFloat_ptr, magma_int_t*, magma_int_t*)

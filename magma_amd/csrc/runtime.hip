@@ -298,3 +298,13 @@ extern "C" void magma_iset_pointer(magma_int_t** output_array, magma_int_t* inpu
     set_pointer_kernel<magma_int_t><<<mgi::ceildiv(batchCount, 256), 256, 0, s>>>(
         output_array, input, lda, row, column, batchSize, batchCount);
 }
+
+extern "C" void magma_sset_pointer(float** output_array, float* input, magma_int_t lda, magma_int_t row,
+                                   magma_int_t column, magma_int_t batch_offset, magma_int_t batchCount,
+                                   magma_queue_t queue)
+{
+    if (batchCount <= 0) return;
+    hipStream_t s = queue ? queue->stream : nullptr;
+    set_pointer_kernel<float><<<mgi::ceildiv(batchCount, 256), 256, 0, s>>>(
+        output_array, input, lda, row, column, batch_offset, batchCount);
+}
