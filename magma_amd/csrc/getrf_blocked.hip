@@ -80,7 +80,10 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
             for (int i = 0; i < R; ++i) {
                 const int r = t + kPanelThreads * i;
                 if (r >= j && r < mloc) {
+                    // idamax: a NaN never wins a comparison, except in the
+                    // first candidate, which keeps the pivot (counted as +inf)
                     T v = fabs(a[i][j]);
+                    if (r == j && v != v) v = T(INFINITY);
                     if (v > best) { best = v; bi = r; }
                 }
             }

@@ -48,6 +48,7 @@ getrf_lds_kernel(int m, int n, T** __restrict__ dA_array, int ldda, int** __rest
             int bi = k;
             for (int i = k + tid; i < m; i += kWave) {
                 T v = fabs(sA[i + k * ld]);
+                if (i == k && v != v) v = T(INFINITY);     // idamax keeps a NaN first entry
                 if (v > best) { best = v; bi = i; }
             }
             lds_wave_argmax(best, bi);

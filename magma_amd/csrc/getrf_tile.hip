@@ -183,8 +183,12 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
                 // constant trip count (the body holds cross-lane operations,
                 // which rule out unrolling with an early exit)
                 if (c >= creal) continue;
-                const double v0 = alive0 ? fabs(x0[c]) : -1.0;
-                const double v1 = alive1 ? fabs(x1[c]) : -1.0;
+                // idamax: NaN never wins, except at the first candidate
+                // position c, which keeps the pivot (counted as +inf)
+                double v0 = alive0 ? fabs(x0[c]) : -1.0;
+                double v1 = alive1 ? fabs(x1[c]) : -1.0;
+                if (pos0 == c && v0 != v0) v0 = INFINITY;
+                if (pos1 == c && v1 != v1) v1 = INFINITY;
                 const double m = wave_max_dpp(fmax(v0, v1));
                 unsigned long long b0 = __ballot(v0 == m);
                 unsigned long long b1 = __ballot(v1 == m);
