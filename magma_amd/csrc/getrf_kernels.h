@@ -25,10 +25,13 @@ struct SizeArgs {
     const int* ld_arr;
     const int* map;          // optional: workgroup index -> matrix index
     __device__ __forceinline__ int mat(int bi) const { return map ? map[bi] : bi; }
+    // each per-matrix array is optional on its own; a null one means the
+    // uniform value
     __device__ __forceinline__ void get(int b, int& mm, int& nn, int& ll) const
     {
-        if (m_arr) { mm = m_arr[b]; nn = n_arr[b]; ll = ld_arr[b]; }
-        else { mm = m; nn = n; ll = ld; }
+        mm = m_arr ? m_arr[b] : m;
+        nn = n_arr ? n_arr[b] : n;
+        ll = ld_arr ? ld_arr[b] : ld;
     }
 };
 

@@ -29,6 +29,7 @@ getrf_lds_kernel(int m, int n, T** __restrict__ dA_array, int ldda, int** __rest
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* sA = reinterpret_cast<T*>(lds_raw);
     __shared__ int s_piv;
+    __shared__ T s_pv;
     const int tid = threadIdx.x;
     const int ld = m + 1;                  // odd stride: row walks spread over banks
     T* A = dA_array[blockIdx.x];
@@ -52,11 +53,13 @@ getrf_lds_kernel(int m, int n, T** __restrict__ dA_array, int ldda, int** __rest
                 if (v > best) { best = v; bi = i; }
             }
             lds_wave_argmax(best, bi);
-            if (tid == 0) s_piv = bi;
+            // the pivot value is published with its index: the row swap below
+            // may move it before every thread has read it from sA
+            if (tid == 0) { s_piv = bi; s_pv = sA[bi + k * ld]; }
         }
         __syncthreads();
         const int p = s_piv;
-        const T pv = sA[p + k * ld];
+        const T pv = s_pv;
         if (tid == 0) ipiv[k] = p + 1 + gbstep;
         if (pv != T(0)) {
             if (p != k) {

@@ -11,12 +11,13 @@
 // Per 16-column panel P (rows k0 = 16P .. n-1):
 //   S1 owners of tile column P stage it in LDS;
 //   S2 one wave factors the panel: LAPACK pivoting (largest |a|, ties to the
-//      lowest current position), reciprocal scaling when |pivot| >= sfmin,
-//      rank-1 updates; rows are not moved during the panel -- their LAPACK
+//      lowest current position), the pivot row read out of its lane into
+//      scalar registers, reciprocal scaling when |pivot| >= sfmin, rank-1
+//      updates; rows are not moved during the panel -- their LAPACK
 //      positions are tracked and the panel is written back in position order;
 //   S3/S4 the panel's row permutation is applied to all other tile columns
 //      through an LDS staging area that holds only the rows that move;
-//   S5 U12 = L11^{-1} A12 by forward substitution, one thread per column;
+//   S5 U12 = L11^{-1} A12 with MFMA, L11^{-1} formed by the panel wave;
 //   S6 A22 -= L21 U12 with v_mfma_f64_16x16x4f64 on the register tiles.
 // With a right-hand side (gesv), B sits in LDS as extra columns: it receives
 // the same interchanges, TRSM and updates (giving L^{-1} P^T B), then a blocked
@@ -75,7 +76,6 @@ struct TileSmem {
     double U[UNION_DOUBLES];                        // laswp staging / U12 / solve scratch
     double B[WITH_B ? NP * LD16 : 1];               // right-hand sides
     double Li[16 * LD16];                           // L11^{-1} of the current panel
-    double prow[16];                                // pivot row broadcast
     int rowmap[NP];                                 // position -> source row (panel-relative)
     int slotof[NP];                                 // moved row -> staging slot, or -1
     int info;
@@ -201,19 +201,16 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
                 }
                 const int ps = (b0 != 0ull) ? 0 : 1;
                 const int pl = (b0 != 0ull) ? first_lane(b0) : first_lane(b1);
-                // the pivot row goes to LDS; every lane reads it back
+                // the pivot row, read out of lane pl into scalar registers
+                double u[16];
                 if (ps == 0) {
-                    if (l == pl) {
 #pragma unroll
-                        for (int c2 = 0; c2 < 16; ++c2)
-                            if (c2 >= c) sm.prow[c2] = x0[c2];
-                    }
+                    for (int c2 = 0; c2 < 16; ++c2)
+                        if (c2 >= c) u[c2] = readlane_d(x0[c2], pl);
                 } else {
-                    if (l == pl) {
 #pragma unroll
-                        for (int c2 = 0; c2 < 16; ++c2)
-                            if (c2 >= c) sm.prow[c2] = x1[c2];
-                    }
+                    for (int c2 = 0; c2 < 16; ++c2)
+                        if (c2 >= c) u[c2] = readlane_d(x1[c2], pl);
                 }
                 const int pos_p = __builtin_amdgcn_readlane(ps ? pos1 : pos0, pl);
                 if (l == 0) ipiv[k0 + c] = k0 + pos_p + 1;
@@ -223,10 +220,6 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
                 pos1 = me1 ? c : (pos1 == c ? pos_p : pos1);
                 alive0 = alive0 && !me0;
                 alive1 = alive1 && !me1;
-                double u[16];
-#pragma unroll
-                for (int c2 = 0; c2 < 16; ++c2)
-                    if (c2 >= c) u[c2] = sm.prow[c2];
                 const double pv = u[c];
                 double l0 = x0[c], l1 = x1[c];
                 if (pv != 0.0) {

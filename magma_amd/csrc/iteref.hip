@@ -589,7 +589,10 @@ magma_int_t iteref_batched(int trans, int n, int nrhs, double** dA_array, int ld
             if (h_active == 0) break;
         }
     } else {
-        // 3b. FGMRES per column (src/dgmres_plu_gpu.cpp:79-316)
+        // 3b. FGMRES per column (src/dgmres_plu_gpu.cpp:79-316). One vector
+        // per system is preconditioned at a time: Rs holds it at stride n.
+        ir_ptrs_kernel<<<g256, 256, 0, stream>>>(Rs_ptr, Rs, (long)n, batch);
+        MGI_LAUNCH_CHECK();
         MGI_HIP_CHECK(hipMemsetAsync(st.done, 0, sizeof(int) * batch, stream));
         MGI_HIP_CHECK(hipMemsetAsync(st.iter, 0, sizeof(int) * batch, stream));
         dim3 g2(std::max(1, std::min(64, ceildiv(n * nrhs, 256))), batch);
