@@ -1,0 +1,278 @@
+// CPU emulation of the HIP subset the library uses, for out-of-bounds hunting
+// under AddressSanitizer. One OS thread per lane; wave intrinsics rendezvous
+// the 64 lanes of a wave; blocks of a grid run one after another.
+#pragma once
+#include <atomic>
+#include <barrier>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct EmuIdx { unsigned x = 0, y = 0, z = 0; };
+extern EmuIdx threadIdx, blockIdx;
+extern EmuIdx blockDim, gridDim;
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+typedef void* hipGraph_t;
+typedef void* hipGraphExec_t;
+typedef void* hipEvent_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
+enum hipMemcpyKind { hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice,
+                     hipMemcpyDefault };
+enum { hipFuncAttributeMaxDynamicSharedMemorySize = 8 };
+enum { hipDeviceAttributeMaxSharedMemoryPerBlock = 74, hipDeviceAttributeMultiprocessorCount = 63 };
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+#define hipStreamNonBlocking 1
+
+namespace emu {
+struct Wave {
+    uint64_t buf[64][8];
+    bool alive[64];
+};
+struct Block {
+    std::vector<Wave> waves;
+    unsigned char* dyn = nullptr;
+};
+extern Block* cur_block;
+extern int cur_lane_global;
+void block_sync();
+void wave_sync();
+void launch_impl(dim3 grid, dim3 block, size_t shm, const std::function<void()>& fn);
+inline unsigned char* dyn_shared() { return cur_block->dyn; }
+inline Wave& wave() { return cur_block->waves[cur_lane_global / 64]; }
+inline int lane() { return cur_lane_global % 64; }
+template <class T>
+inline void put(const T& v)
+{
+    static_assert(sizeof(T) <= 64, "wave exchange value too large");
+    std::memcpy(wave().buf[lane()], &v, sizeof(T));
+}
+template <class T>
+inline T get(int l)
+{
+    T v;
+    std::memcpy(&v, wave().buf[l], sizeof(T));
+    return v;
+}
+inline void wsync() { wave_sync(); }
+}  // namespace emu
+
+inline void __syncthreads() { emu::block_sync(); }
+
+template <class T>
+inline T emu_shfl(T v, int src)
+{
+    emu::put(v);
+    emu::wsync();
+    T r = emu::get<T>(src & 63);
+    emu::wsync();
+    return r;
+}
+inline int __shfl_xor(int v, int mask, int = 64) { return emu_shfl(v, emu::lane() ^ mask); }
+inline float __shfl_xor(float v, int mask, int = 64) { return emu_shfl(v, emu::lane() ^ mask); }
+inline int __shfl(int v, int src, int = 64) { return emu_shfl(v, src); }
+inline float __shfl(float v, int src, int = 64) { return emu_shfl(v, src); }
+inline unsigned long long __ballot(int pred)
+{
+    emu::put<int>(pred ? 1 : 0);
+    emu::wave().alive[emu::lane()] = true;
+    emu::wsync();
+    unsigned long long m = 0;
+    for (int l = 0; l < 64; ++l)
+        if (emu::wave().alive[l] && emu::get<int>(l)) m |= 1ull << l;
+    emu::wsync();
+    return m;
+}
+inline int __builtin_amdgcn_readlane(int v, int l) { return emu_shfl(v, l); }
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound)
+{
+    (void)bank_mask;
+    (void)bound;
+    const int l = emu::lane();
+    int s;
+    if (ctrl < 0x100) s = (l & ~3) + ((ctrl >> (2 * (l & 3))) & 3);
+    else if (ctrl == 0x140) s = (l & ~15) + (15 - (l & 15));
+    else if (ctrl == 0x141) s = (l & ~7) + (7 - (l & 7));
+    else if (ctrl == 0x142) s = (l & ~15) - 1;     // lane 15 of the previous row
+    else if (ctrl == 0x143) s = 31;
+    else {
+        std::fprintf(stderr, "emu: unsupported dpp ctrl %#x\n", ctrl);
+        std::abort();
+    }
+    const int r = emu_shfl(src, s < 0 ? l : s);
+    const int row = l >> 4;
+    if (!((row_mask >> row) & 1)) return old;
+    if (s < 0) return old;
+    return r;
+}
+inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
+inline int __ffsll(long long m) { return __builtin_ffsll(m); }
+inline int __double2loint(double d) { int64_t b; std::memcpy(&b, &d, 8); return (int)(uint32_t)b; }
+inline int __double2hiint(double d) { int64_t b; std::memcpy(&b, &d, 8); return (int)(b >> 32); }
+inline double __hiloint2double(int hi, int lo)
+{
+    uint64_t b = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicSub(int* p, int v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
+
+template <class A, class B>
+inline auto min(A a, B b) -> typename std::common_type<A, B>::type { return a < b ? a : b; }
+template <class A, class B>
+inline auto max(A a, B b) -> typename std::common_type<A, B>::type { return a > b ? a : b; }
+using std::fabs;
+using std::fma;
+using std::fmax;
+using std::fmin;
+using std::isfinite;
+using std::sqrt;
+
+// MFMA: operands exchanged across the wave; layouts of gfx950.
+typedef double emu_d4 __attribute__((ext_vector_type(4)));
+typedef float emu_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 emu_bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 emu_h8 __attribute__((ext_vector_type(8)));
+
+inline emu_d4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, emu_d4 c, int, int, int)
+{
+    struct AB { double a, b; };
+    emu::put(AB{a, b});
+    emu::wsync();
+    const int l = emu::lane();
+    emu_d4 out = c;
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * r + l / 16, col = l % 16;
+        double acc = c[r];
+        for (int k = 0; k < 4; ++k) acc += emu::get<AB>(k * 16 + row).a * emu::get<AB>(k * 16 + col).b;
+        out[r] = acc;
+    }
+    emu::wsync();
+    return out;
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, emu_f4 c, int, int, int)
+{
+    struct AB { float a, b; };
+    emu::put(AB{a, b});
+    emu::wsync();
+    const int l = emu::lane();
+    emu_f4 out = c;
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * (l / 16) + r, col = l % 16;
+        float acc = c[r];
+        for (int k = 0; k < 4; ++k) acc += emu::get<AB>(k * 16 + row).a * emu::get<AB>(k * 16 + col).b;
+        out[r] = acc;
+    }
+    emu::wsync();
+    return out;
+}
+template <class V8>
+inline emu_f4 emu_mfma_x32(V8 a, V8 b, emu_f4 c)
+{
+    struct AB { float a[8], b[8]; };
+    AB ab;
+    for (int v = 0; v < 8; ++v) { ab.a[v] = (float)a[v]; ab.b[v] = (float)b[v]; }
+    emu::put(ab);
+    emu::wsync();
+    const int l = emu::lane();
+    emu_f4 out = c;
+    for (int r = 0; r < 4; ++r) {
+        const int row = 4 * (l / 16) + r, col = l % 16;
+        float acc = c[r];
+        for (int q = 0; q < 4; ++q)
+            for (int v = 0; v < 8; ++v) acc += emu::get<AB>(q * 16 + row).a[v] * emu::get<AB>(q * 16 + col).b[v];
+        out[r] = acc;
+    }
+    emu::wsync();
+    return out;
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf8 a, emu_bf8 b, emu_f4 c, int, int, int)
+{
+    return emu_mfma_x32(a, b, c);
+}
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_f16(emu_h8 a, emu_h8 b, emu_f4 c, int, int, int)
+{
+    return emu_mfma_x32(a, b, c);
+}
+
+// Runtime API (host memory stands in for device memory).
+hipError_t hipMalloc(void** p, size_t bytes);
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc((void**)p, bytes); }
+hipError_t hipFree(void* p);
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned = 0);
+hipError_t hipHostFree(void* p);
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return 0; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind, hipStream_t = nullptr)
+{
+    std::memmove(d, s, n);
+    return 0;
+}
+inline hipError_t hipMemcpy2DAsync(void* d, size_t dp, const void* s, size_t sp, size_t w, size_t h,
+                                   hipMemcpyKind, hipStream_t = nullptr)
+{
+    for (size_t r = 0; r < h; ++r) std::memmove((char*)d + r * dp, (const char*)s + r * sp, w);
+    return 0;
+}
+inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t = nullptr) { std::memset(d, v, n); return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipDeviceSynchronize() { return 0; }
+inline hipError_t hipGetLastError() { return 0; }
+inline hipError_t hipPeekAtLastError() { return 0; }
+inline const char* hipGetErrorString(hipError_t) { return "emu"; }
+inline hipError_t hipSetDevice(int) { return 0; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return 0; }
+inline hipError_t hipGetDeviceCount(int* c) { *c = 1; return 0; }
+inline hipError_t hipDeviceGetAttribute(int* v, int attr, int)
+{
+    *v = (attr == hipDeviceAttributeMaxSharedMemoryPerBlock) ? 163840 : 256;
+    return 0;
+}
+inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }
+inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)0x1; return 0; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)0x1; return 0; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return 1; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return 1; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return 1; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 1; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 0; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return 0; }
+
+#define EMU_LAUNCH(GRID, BLOCK, ...) emu::Launch(dim3(GRID), dim3(BLOCK), ##__VA_ARGS__)
+namespace emu {
+struct Launch {
+    dim3 g, b;
+    size_t shm;
+    Launch(dim3 g_, dim3 b_, size_t shm_ = 0, hipStream_t = nullptr) : g(g_), b(b_), shm(shm_) {}
+    void operator<<(const std::function<void()>& fn) const { launch_impl(g, b, shm, fn); }
+};
+}  // namespace emu
