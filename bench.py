@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=20000,
                     help="systems timed with host LAPACK for the CPU baseline (0 = skip)")
     ap.add_argument("--seed", type=int, default=20250101)
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="also time BF16-LU + FP64 refinement against FP64 dgesv on 2,000 x 256^2 (0 = skip)")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="also time the differentiable solve (forward + backward) on 8,192 x 64^2 (0 = skip)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this many MiB once on the device (a kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -112,6 +116,112 @@ def cpu_baseline(A_host, B_host, n, nrhs, min_secs=10.0):
                    f"max tester backward error {float(np.nanmax(be)):.2e}"),
         "gflops": solved * (flops_getrf(n) + flops_getrs(n, nrhs)) / secs / 1e9,
     }
+
+
+def svd_geo_batch(torch, batch, n, cond, gen, dev):
+    """A = U diag(sigma) V^T with Haar U, V and sigma_i = cond^(-(i-1)/(n-1))
+    (testing/magma_generate.cpp, matrix svd_geo), generated on the device."""
+    def haar():
+        Z = torch.randn(batch, n, n, dtype=torch.float64, device=dev, generator=gen)
+        out = torch.empty_like(Z)
+        for c in range(0, batch, 125):      # the batched QR's workspace is sized per call
+            Q, R = torch.linalg.qr(Z[c:c + 125])
+            out[c:c + 125] = Q * torch.sign(torch.diagonal(R, dim1=1, dim2=2))[:, None, :]
+        return out
+    sig = cond ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+    U = haar() * sig[None, None, :]
+    Vt = haar().transpose(1, 2).contiguous()
+    return torch.bmm(U, Vt)
+
+
+def backward_errors(torch, A_cm, X_cm, B_cm):
+    """||B - A X||_inf / (||A||_inf ||X||_inf) per system; all three in the
+    column-major (batch, ncols, ld) layout of the C interface."""
+    A = A_cm.transpose(1, 2)
+    X = X_cm.transpose(1, 2)
+    R = B_cm.transpose(1, 2) - torch.bmm(A, X.contiguous())
+    rn = R.abs().sum(2).amax(1)
+    an = A.abs().sum(2).amax(1)
+    xn = X.abs().sum(2).amax(1)
+    return rn / (an * xn)
+
+
+def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n=256, nrhs=1, cond=1e2):
+    """BF16-update LU + FP64 refinement (FGMRES) against FP64 dgesv_batched on
+    the same batch of svd_geo systems: solves/s, FP64-credited GFLOP/s,
+    refinement steps and the worst backward / forward errors."""
+    from magma_amd import iteref
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A = svd_geo_batch(torch, batch, n, cond, gen, dev)          # read as column-major
+    B = torch.rand((batch, nrhs, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+    flop = batch * (flops_getrf(n) + flops_getrs(n, nrhs))
+    res = {"workload": f"{batch} x {n}x{n} svd_geo cond {cond:g}, nrhs={nrhs}", "steps": steps}
+
+    # FP64 reference path: every step factors its own copy
+    copies = [(A.clone(), B.clone()) for _ in range(steps + 1)]
+    mb.dgesv_batched(*copies[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        mb.dgesv_batched(*copies[k])
+    torch.cuda.synchronize()
+    t64 = (time.perf_counter() - t0) / steps
+    X64 = copies[1][1]
+    be64 = float(backward_errors(torch, A, X64, B).max())
+    del copies
+
+    out = {}
+    for facto in ("bf16", "fp16"):
+        X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver="gmres", itermax=100)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver="gmres", itermax=100)
+        torch.cuda.synchronize()
+        tir = (time.perf_counter() - t0) / steps
+        be = float(backward_errors(torch, A, X, B).max())
+        fe = float(((X - X64).abs().amax(dim=(1, 2)) / X64.abs().amax(dim=(1, 2))).max())
+        itc = it.cpu()
+        out[facto] = {"solves_per_s": round(batch / tir, 1), "ms": round(tir * 1e3, 3),
+                      "gflops_fp64_credited": round(flop / tir / 1e9, 1),
+                      "iters_mean": round(float(itc.clamp(min=0).double().mean()), 2),
+                      "fallbacks": int((itc < 0).sum()), "backward_error_max": be,
+                      "forward_error_vs_fp64_max": fe, "speedup_vs_fp64": round(t64 / tir, 3)}
+    res.update({"fp64_dgesv": {"solves_per_s": round(batch / t64, 1), "ms": round(t64 * 1e3, 3),
+                               "gflops": round(flop / t64 / 1e9, 1), "backward_error_max": be64},
+                "lu_gmres_ir": out})
+    return res
+
+
+def adjoint_leg(torch, dev, seed, steps=5, batch=8192, n=64, nrhs=8):
+    """Forward + backward of the differentiable solve, FP64 and FP32."""
+    from magma_amd.autograd import BatchedSolve
+    res = {"workload": f"{batch} x {n}x{n} (U[-1,1] + 8I), nrhs={nrhs}, loss = sum(W * X)", "steps": steps}
+    gen = torch.Generator(device=dev)
+    for dt, name in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        gen.manual_seed(seed)
+        A = (torch.rand((batch, n, n), dtype=dt, device=dev, generator=gen) * 2 - 1
+             + 8 * torch.eye(n, dtype=dt, device=dev))
+        B = torch.rand((batch, n, nrhs), dtype=dt, device=dev, generator=gen) * 2 - 1
+        W = torch.rand((batch, n, nrhs), dtype=dt, device=dev, generator=gen) * 2 - 1
+
+        def step():
+            a = A.detach().requires_grad_(True)
+            b = B.detach().requires_grad_(True)
+            X, _ = BatchedSolve.apply(a, b)
+            ga, gb = torch.autograd.grad(X, (a, b), W)
+            return X, ga, gb
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            X, ga, gb = step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        res[name] = {"systems_per_s": round(batch / t, 1), "ms_fwd_bwd": round(t * 1e3, 3)}
+    return res
 
 
 def main():
@@ -241,14 +351,28 @@ def main():
         "fp64_frac": round(flop / ev_avg_s / 1e12 / FP64_PEAK_TFLOPS, 4),
     }
 
-    cpu = None
+    A_host = B_host = None
     if args.cpu_sample > 0:
-        print(f"[bench] cpu baseline on {args.cpu_sample} systems", file=sys.stderr, flush=True)
         m = min(args.cpu_sample, batch)
         refresh(0)
         torch.cuda.synchronize()
         A_host = sets[0]["A"][:m].cpu().numpy()
         B_host = sets[0]["B"][:m].cpu().numpy()
+    del sets
+    torch.cuda.empty_cache()
+
+    mixed = None
+    if args.mixed:
+        print("[bench] mixed-precision leg", file=sys.stderr, flush=True)
+        mixed = mixed_precision_leg(torch, mb, lib, queue, dev, args.seed)
+    adjoint = None
+    if args.adjoint:
+        print("[bench] adjoint leg", file=sys.stderr, flush=True)
+        adjoint = adjoint_leg(torch, dev, args.seed)
+
+    cpu = None
+    if A_host is not None:
+        print(f"[bench] cpu baseline on {A_host.shape[0]} systems", file=sys.stderr, flush=True)
         cpu = cpu_baseline(A_host, B_host, n, nrhs)
 
     out = {
@@ -272,6 +396,8 @@ def main():
         "nonzero_info": n_info,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,8 +1,15 @@
 // Square batched LU (and LU + solve) for n <= 128 with the whole matrix held
 // in registers as 16x16 FP64 tiles, one workgroup (4 waves) per matrix.
 //
-// Data layout. Tile (I,J) covers rows 16I..16I+15, columns 16J..16J+15 and is
-// owned by wave (I%2)*2 + (J%2). Within a tile, lane l holds
+// Roles. Waves 0..2 ("tile waves") hold the tiles: tile (I,J) sits in wave
+// (I*NT + J) % 3, so every block row, block column and trailing submatrix is
+// spread over the three. Wave 3 (the "panel wave") holds no tiles: it factors
+// each 16-column panel, keeps the right-hand sides and solves the diagonal
+// blocks of the back substitution. The two roles run separate programs with
+// the same sequence of workgroup barriers, so the tile registers are not live
+// in the panel wave's code (no register spills at n = 128).
+//
+// Tile layout. Within a tile, lane l holds
 //     element (16I + l%16, 16J + 4r + l/16) in register r = 0..3,
 // which is the accumulator layout of v_mfma_f64_16x16x4f64 for the transposed
 // tile (so an update C^T += U^T L^T lands in place) and lets 16 lanes read 128
@@ -10,14 +17,15 @@
 //
 // Per 16-column panel P (rows k0 = 16P .. n-1):
 //   S1 owners of tile column P stage it in LDS;
-//   S2 one wave factors the panel: LAPACK pivoting (largest |a|, ties to the
-//      lowest current position), the pivot row read out of its lane into
-//      scalar registers, reciprocal scaling when |pivot| >= sfmin, rank-1
-//      updates; rows are not moved during the panel -- their LAPACK
+//   S2 the panel wave factors the panel: LAPACK pivoting (largest |a|, ties
+//      to the lowest current position), the pivot row read out of its lane
+//      into scalar registers, reciprocal scaling when |pivot| >= sfmin,
+//      rank-1 updates; rows are not moved during the panel -- their LAPACK
 //      positions are tracked and the panel is written back in position order;
+//      then L11^{-1};
 //   S3/S4 the panel's row permutation is applied to all other tile columns
 //      through an LDS staging area that holds only the rows that move;
-//   S5 U12 = L11^{-1} A12 with MFMA, L11^{-1} formed by the panel wave;
+//   S5 U12 = L11^{-1} A12 with MFMA;
 //   S6 A22 -= L21 U12 with v_mfma_f64_16x16x4f64 on the register tiles.
 // With a right-hand side (gesv), B sits in LDS as extra columns: it receives
 // the same interchanges, TRSM and updates (giving L^{-1} P^T B), then a blocked
@@ -42,6 +50,13 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kTileThreads = 256;
 constexpr int kTileWaves = 4;
+constexpr int kTileOwners = 3;      // waves 0..2 hold tiles, wave 3 factors panels
+
+// Matrices are reached through pointer arrays, so the compiler sees generic
+// pointers; naming the global address space lets it use global loads with a
+// scalar base and a 32-bit lane offset instead of 64-bit flat addresses.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* as_global(double* p) { return (gdouble*)p; }
 
 __device__ __forceinline__ double wave_max_d(double v)
 {
@@ -81,92 +96,262 @@ struct TileSmem {
     int info;
 };
 
+// Tile slots of a tile wave: slot k of wave w holds tile q = 3k + w (row-major
+// tile index), i.e. tile (q / NT, q % NT). Consecutive tiles rotate over the
+// three tile waves, so every block row, block column and trailing submatrix
+// is shared about evenly.
+template <int NT>
+struct TileMap {
+    static constexpr int KT = (NT * NT + kTileOwners - 1) / kTileOwners;
+    __device__ static constexpr bool valid(int w, int k) { return kTileOwners * k + w < NT * NT; }
+    __device__ static constexpr int row(int w, int k) { return (kTileOwners * k + w) / NT; }
+    __device__ static constexpr int col(int w, int k) { return (kTileOwners * k + w) % NT; }
+};
+
+// Barrier schedule shared by the two wave programs below (each __syncthreads
+// in one program has a partner in the other, in the same order):
+//   load | per panel: S1 | S2 | S3 | S4a | S4b | S5 | S6 | back substitution:
+//   2 per block row.
 template <int NT, bool WITH_B>
-__global__ void __launch_bounds__(kTileThreads, 2)
-getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict__ ipiv_array,
-                  int* __restrict__ info_array, double** __restrict__ dB_array)
+struct TileCtx {
+    TileSmem<NT, WITH_B>& sm;
+    int n, nrhs, ldda, lddb;
+    double* A;
+    int* ipiv;
+    double* B;
+    int l, lr, lq;
+};
+
+// ---------------------------------------------------------------------------
+// Tile waves (0..2): the matrix in MFMA accumulator layout, row interchanges,
+// U12 and the trailing updates, the factors' write-back, back-substitution
+// partial products.
+// ---------------------------------------------------------------------------
+template <int NT, bool WITH_B, int W>
+__device__ __forceinline__ void tile_wave_main(const TileCtx<NT, WITH_B>& cx)
 {
+    constexpr int w = W;                  // tile coordinates are compile-time constants
     constexpr int NP = 16 * NT;
-    constexpr int NTW = (NT + 1) / 2;
+    constexpr int KT = TileMap<NT>::KT;
     constexpr int LD16 = TileSmem<NT, WITH_B>::LD16;
     constexpr int ULD = TileSmem<NT, WITH_B>::ULD;
     constexpr int SLD = TileSmem<NT, WITH_B>::SLD;
-    constexpr int UCOLS = TileSmem<NT, WITH_B>::UCOLS;
-    __shared__ TileSmem<NT, WITH_B> sm;
+    auto& sm = cx.sm;
+    const int n = cx.n, ldda = cx.ldda, lr = cx.lr, lq = cx.lq;
+    double* A = cx.A;
 
-    const int t = threadIdx.x;
-    const int w = t / kWave;
-    const int l = t % kWave;
-    const int wr = w >> 1, wc = w & 1;
-    const int lr = l % 16;            // row within a tile
-    const int lq = l / 16;            // column group within a tile
-    const int mat = ta.map ? ta.map[blockIdx.x] : blockIdx.x;
-    const int n = ta.n_arr ? ta.n_arr[mat] : ta.n;
-    const int nrhs = ta.nrhs_arr ? ta.nrhs_arr[mat] : ta.nrhs;
-    const int ldda = ta.ldda_arr ? ta.ldda_arr[mat] : ta.ldda;
-    const int lddb = ta.lddb_arr ? ta.lddb_arr[mat] : ta.lddb;
-    if (n <= 0) return;                   // empty system: nothing to factor
-    double* A = dA_array[mat];
-    int* ipiv = ipiv_array[mat];
-
-    // ---- load tiles (padding = identity) ----
-    double4_t acc[NTW][NTW];
+    // A tile is addressed as a wave-uniform base plus one of four 32-bit lane
+    // offsets shared by all tiles (ldda < 2^27 keeps them in range).
+    int off[4];
 #pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-        const int row = 16 * (2 * i + wr) + lr;
+    for (int r = 0; r < 4; ++r) off[r] = lr + (4 * r + lq) * ldda;
+    double4_t acc[KT];
 #pragma unroll
-        for (int j = 0; j < NTW; ++j) {
+    for (int k = 0; k < KT; ++k) {
+        if (!TileMap<NT>::valid(w, k)) continue;
+        const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+        const gdouble* At = as_global(A) + 16 * I + (size_t)(16 * J) * ldda;
+        if (16 * I + 16 <= n && 16 * J + 16 <= n) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[k][r] = At[off[r]];
+        } else {
+            const int row = 16 * I + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int col = 16 * (2 * j + wc) + 4 * r + lq;
-                double v = 0.0;
-                if (2 * i + wr < NT && 2 * j + wc < NT) {
-                    if (row < n && col < n) v = A[row + (size_t)col * ldda];
-                    else v = (row == col) ? 1.0 : 0.0;
-                }
-                acc[i][j][r] = v;
+                const int col = 16 * J + 4 * r + lq;
+                acc[k][r] = (row < n && col < n) ? At[off[r]] : (row == col ? 1.0 : 0.0);
             }
         }
     }
-    if constexpr (WITH_B) {
-        const double* B = dB_array[mat];
-        for (int idx = t; idx < NP * 16; idx += kTileThreads) {
-            const int c = idx / NP, row = idx % NP;
-            sm.B[row * LD16 + c] = (row < n && c < nrhs) ? B[row + (size_t)c * lddb] : 0.0;
+    __syncthreads();                                              // load
+
+    const int npan = (n + 15) / 16;
+    for (int P = 0; P < npan; ++P) {
+        // S1: the owners of tile column P stage it (rows >= 16P) in sm.L
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            if (!TileMap<NT>::valid(w, k)) continue;
+            const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+            if (J != P || I < P) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sm.L[(16 * (I - P) + lr) * LD16 + 4 * r + lq] = acc[k][r];
+        }
+        __syncthreads();                                          // S1
+        __syncthreads();                                          // S2 (panel wave)
+
+        // S3: stage the rows that move; reload the factored panel
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            if (!TileMap<NT>::valid(w, k)) continue;
+            const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+            if (I < P) continue;
+            const int rr = 16 * (I - P) + lr;
+            if (J == P) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[k][r] = sm.L[rr * LD16 + 4 * r + lq];
+            } else {
+                const int slot = sm.slotof[rr];
+                if (slot >= 0) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) sm.U[slot * SLD + 16 * J + 4 * r + lq] = acc[k][r];
+                }
+            }
+        }
+        __syncthreads();                                          // S3
+
+        // S4a: apply the row permutation from the staging area
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            if (!TileMap<NT>::valid(w, k)) continue;
+            const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+            if (I < P || J == P) continue;
+            const int rr = 16 * (I - P) + lr;
+            const int src = sm.rowmap[rr];
+            if (src != rr) {
+                const int slot = sm.slotof[src];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[k][r] = sm.U[slot * SLD + 16 * J + 4 * r + lq];
+            }
+        }
+        __syncthreads();                                          // S4a
+
+        // S4b: block row P right of the panel into sm.U
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            if (!TileMap<NT>::valid(w, k)) continue;
+            const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+            if (I != P || J <= P) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = acc[k][r];
+        }
+        __syncthreads();                                          // S4b
+
+        // S5: U12 = L11^{-1} A12 on the owners of block row P
+        // (U12^T = A12^T L11^{-T}: the result lands in the tile layout)
+        {
+            double lb[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) lb[kk] = sm.Li[lr * LD16 + 4 * kk + lq];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                if (!TileMap<NT>::valid(w, k)) continue;
+                const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+                if (I != P || J <= P) continue;
+                double4_t uacc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double a12 = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
+                    uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lb[kk], uacc, 0, 0, 0);
+                }
+                acc[k] = uacc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = uacc[r];
+            }
+        }
+        __syncthreads();                                          // S5
+
+        // S6: trailing update; lane l uses -L(16I + l%16, 4kk + l/16) and
+        // U12(4kk + l/16, 16J + l%16)
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            if (!TileMap<NT>::valid(w, k)) continue;
+            const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+            if (I <= P || J <= P) continue;
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double uf = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
+                const double lf = -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq];
+                acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(uf, lf, acc[k], 0, 0, 0);
+            }
+        }
+        __syncthreads();                                          // S6
+    }
+
+    // the factors
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        if (!TileMap<NT>::valid(w, k)) continue;
+        const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+        gdouble* At = as_global(A) + 16 * I + (size_t)(16 * J) * ldda;
+        if (16 * I + 16 <= n && 16 * J + 16 <= n) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) At[off[r]] = acc[k][r];
+        } else {
+            const int row = 16 * I + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = 16 * J + 4 * r + lq;
+                if (row < n && col < n) At[off[r]] = acc[k][r];
+            }
         }
     }
-    if (t == 0) sm.info = 0;
-    __syncthreads();
+
+    if constexpr (WITH_B) {
+        // back substitution, block rows bottom-up: each tile wave sums its
+        // tiles' share of U_IJ X_J (J > I) into its own 16x16 slice of T; the
+        // owner of U_II copies it to slice 3; the panel wave solves.
+        double* T = sm.U;                         // [4][16][LD16]
+        const int nb_real = (n + 15) / 16;
+        for (int I = nb_real - 1; I >= 0; --I) {
+            double4_t part = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                if (!TileMap<NT>::valid(w, k)) continue;
+                const int Ik = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
+                if (Ik != I) continue;
+                if (J > I) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const double xb = sm.B[(16 * J + 4 * r + lq) * LD16 + lr];
+                        part = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[k][r], xb, part, 0, 0, 0);
+                    }
+                } else if (J == I) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) T[(3 * 16 + lr) * LD16 + 4 * r + lq] = acc[k][r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) T[(w * 16 + 4 * r + lq) * LD16 + lr] = part[r];
+            __syncthreads();                                      // partials ready
+            __syncthreads();                                      // block row solved
+        }
+    }
+    (void)NP;
+}
+
+// ---------------------------------------------------------------------------
+// Panel wave (3): the right-hand sides, the panel factorisation with LAPACK
+// pivoting, L11^{-1}, B's interchanges / TRSM / updates and the 16x16
+// triangular solves of the back substitution.
+// ---------------------------------------------------------------------------
+template <int NT, bool WITH_B>
+__device__ __forceinline__ void panel_wave_main(const TileCtx<NT, WITH_B>& cx)
+{
+    constexpr int NP = 16 * NT;
+    constexpr int LD16 = TileSmem<NT, WITH_B>::LD16;
+    constexpr int ULD = TileSmem<NT, WITH_B>::ULD;
+    auto& sm = cx.sm;
+    const int n = cx.n, l = cx.l, lr = cx.lr, lq = cx.lq;
+    int* ipiv = cx.ipiv;
+
+    if constexpr (WITH_B) {
+        for (int idx = l; idx < NP * 16; idx += kWave) {
+            const int c = idx / NP, row = idx % NP;
+            sm.B[row * LD16 + c] = (row < n && c < cx.nrhs) ? cx.B[row + (size_t)c * cx.lddb] : 0.0;
+        }
+    }
+    if (l == 0) sm.info = 0;
+    __syncthreads();                                              // load
 
     const int npan = (n + 15) / 16;
     for (int P = 0; P < npan; ++P) {
         const int k0 = 16 * P;
         const int mp = NP - k0;                       // panel rows (incl. padding)
-        const int pw = P % kTileWaves;                // wave that factors this panel
+        __syncthreads();                                          // S1 (tile waves)
 
-        // ---- S1: stage tile column P (rows >= k0) into sm.L ----
-        if (wc == (P & 1)) {
-#pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                const int I = 2 * i + wr;
-                if (I < NT && I >= P) {
-                    const int j = P >> 1;
-#pragma unroll
-                    for (int jj = 0; jj < NTW; ++jj) {
-                        if (jj != j) continue;
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            sm.L[(16 * (I - P) + lr) * LD16 + 4 * r + lq] = acc[i][jj][r];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---- S2: panel factorisation by one wave ----
-        // Lane l holds panel rows l and l+64 (the second only while the panel
-        // is taller than 64 rows). A row leaves the candidate set once chosen.
-        if (w == pw) {
+        // S2: lane l holds panel rows l and l+64 (the second only while the
+        // panel is taller than 64 rows); a row leaves the candidates once chosen
+        {
             const bool has1 = mp > 64;
             double x0[16], x1[16];
 #pragma unroll
@@ -266,152 +451,73 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
             if (l < mp) sm.slotof[l] = mv0 ? __popcll(m0 & below) : -1;
             if (has1 && l + 64 < mp) sm.slotof[l + 64] = mv1 ? (__popcll(m0) + __popcll(m1 & below)) : -1;
             if (l == 0 && info_local != 0 && sm.info == 0) sm.info = info_local;
-
+        }
+        // the rows just written by other lanes are read back below
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        {
             // L11^{-1} by forward substitution on the unit columns: lane
             // 4j + q owns column j, rows q, q+4, q+8, q+12; the solved entry
             // of step k is broadcast within the quad.
-            {
-                const int j = l >> 2, q = l & 3;
-                double xi[4];
+            const int j = l >> 2, q = l & 3;
+            double xi[4];
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) xi[s4] = (q + 4 * s4 == j) ? 1.0 : 0.0;
+            for (int s4 = 0; s4 < 4; ++s4) xi[s4] = (q + 4 * s4 == j) ? 1.0 : 0.0;
 #pragma unroll
-                for (int k = 0; k < 15; ++k) {
-                    double xk;
-                    switch (k & 3) {
-                        case 0: xk = quad_bcast_d<0>(xi[k >> 2]); break;
-                        case 1: xk = quad_bcast_d<1>(xi[k >> 2]); break;
-                        case 2: xk = quad_bcast_d<2>(xi[k >> 2]); break;
-                        default: xk = quad_bcast_d<3>(xi[k >> 2]); break;
-                    }
-#pragma unroll
-                    for (int s4 = 0; s4 < 4; ++s4) {
-                        const int i = q + 4 * s4;
-                        const double coef = (i > k) ? sm.L[i * LD16 + k] : 0.0;
-                        xi[s4] = fma(-coef, xk, xi[s4]);
-                    }
+            for (int k = 0; k < 15; ++k) {
+                double xk;
+                switch (k & 3) {
+                    case 0: xk = quad_bcast_d<0>(xi[k >> 2]); break;
+                    case 1: xk = quad_bcast_d<1>(xi[k >> 2]); break;
+                    case 2: xk = quad_bcast_d<2>(xi[k >> 2]); break;
+                    default: xk = quad_bcast_d<3>(xi[k >> 2]); break;
                 }
 #pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) sm.Li[(q + 4 * s4) * LD16 + j] = xi[s4];
+                for (int s4 = 0; s4 < 4; ++s4) {
+                    const int i = q + 4 * s4;
+                    const double coef = (i > k) ? sm.L[i * LD16 + k] : 0.0;
+                    xi[s4] = fma(-coef, xk, xi[s4]);
+                }
             }
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) sm.Li[(q + 4 * s4) * LD16 + j] = xi[s4];
         }
-        __syncthreads();
+        __syncthreads();                                          // S2
 
-        // ---- S3: stage rows that move; reload the factored panel ----
-#pragma unroll
-        for (int i = 0; i < NTW; ++i) {
-            const int I = 2 * i + wr;
-            if (I >= NT || I < P) continue;
-            const int rr = 16 * (I - P) + lr;
-            const int slot = sm.slotof[rr];
-#pragma unroll
-            for (int j = 0; j < NTW; ++j) {
-                const int J = 2 * j + wc;
-                if (J >= NT) continue;
-                if (J == P) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.L[rr * LD16 + 4 * r + lq];
-                } else if (slot >= 0) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) sm.U[slot * SLD + 16 * J + 4 * r + lq] = acc[i][j][r];
-                }
-            }
-        }
-        double btmp[WITH_B ? (NP * 16 + kTileThreads - 1) / kTileThreads : 1];
         if constexpr (WITH_B) {
+            // S3: gather B's rows in their new order
+            constexpr int BQ = (NP * 16 + kWave - 1) / kWave;
+            double btmp[BQ];
 #pragma unroll
-            for (int q = 0; q < (NP * 16 + kTileThreads - 1) / kTileThreads; ++q) {
-                const int idx = t + q * kTileThreads;
+            for (int q = 0; q < BQ; ++q) {
+                const int idx = l + q * kWave;
                 const int i = idx / 16, c = idx % 16;
-                if (i < mp) {
-                    const int src = sm.rowmap[i];
-                    btmp[q] = sm.B[(k0 + src) * LD16 + c];
-                }
+                if (i < mp) btmp[q] = sm.B[(k0 + sm.rowmap[i]) * LD16 + c];
             }
-        }
-        __syncthreads();
-
-        // ---- S4a: apply the row permutation from the staging area ----
+            __syncthreads();                                      // S3
+            // S4a
 #pragma unroll
-        for (int i = 0; i < NTW; ++i) {
-            const int I = 2 * i + wr;
-            if (I >= NT || I < P) continue;
-            const int rr = 16 * (I - P) + lr;
-            const int src = sm.rowmap[rr];
-            if (src == rr) continue;
-            const int slot = sm.slotof[src];
-#pragma unroll
-            for (int j = 0; j < NTW; ++j) {
-                const int J = 2 * j + wc;
-                if (J >= NT || J == P) continue;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[i][j][r] = sm.U[slot * SLD + 16 * J + 4 * r + lq];
-            }
-        }
-        if constexpr (WITH_B) {
-#pragma unroll
-            for (int q = 0; q < (NP * 16 + kTileThreads - 1) / kTileThreads; ++q) {
-                const int idx = t + q * kTileThreads;
+            for (int q = 0; q < BQ; ++q) {
+                const int idx = l + q * kWave;
                 const int i = idx / 16, c = idx % 16;
                 if (i < mp) sm.B[(k0 + i) * LD16 + c] = btmp[q];
             }
-        }
-        __syncthreads();
-
-        // ---- S4b: block row P right of the panel (and B) into sm.U ----
-        if (wr == (P & 1)) {
+            __syncthreads();                                      // S4a
+            // S4b: B's panel rows beside U12
 #pragma unroll
-            for (int j = 0; j < NTW; ++j) {
-                const int J = 2 * j + wc;
-                if (J >= NT || J <= P) continue;
-#pragma unroll
-                for (int i = 0; i < NTW; ++i) {
-                    if (2 * i + wr != P) continue;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = acc[i][j][r];
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int i = 4 * q + lq;
+                sm.U[i * ULD + NP + lr] = sm.B[(k0 + i) * LD16 + lr];
             }
-        }
-        if constexpr (WITH_B) {
-            const int i = t / 16, c = t % 16;
-            sm.U[i * ULD + NP + c] = sm.B[(k0 + i) * LD16 + c];
-        }
-        __syncthreads();
-
-        // ---- S5: U12 = L11^{-1} A12 with MFMA on the owners of block row P ----
-        // (U12^T = A12^T L11^{-T}: the result lands in the tile layout.)
-        if (wr == (P & 1)) {
-            double lb[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) lb[kk] = sm.Li[lr * LD16 + 4 * kk + lq];
-#pragma unroll
-            for (int j = 0; j < NTW; ++j) {
-                const int J = 2 * j + wc;
-                if (J >= NT || J <= P) continue;
-#pragma unroll
-                for (int i = 0; i < NTW; ++i) {
-                    if (2 * i + wr != P) continue;
-                    double4_t uacc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        const double a12 = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
-                        uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lb[kk], uacc, 0, 0, 0);
-                    }
-                    acc[i][j] = uacc;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) sm.U[lr * ULD + 16 * J + 4 * r + lq] = uacc[r];
-                }
-            }
-        }
-        if constexpr (WITH_B) {
-            // the right-hand sides' panel rows, on a wave outside block row P
-            if (w == (((P & 1) ^ 1) << 1)) {
+            __syncthreads();                                      // S4b
+            // S5: Y_P = L11^{-1} B_P
+            {
                 double4_t uacc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     const double a12 = sm.U[(4 * kk + lq) * ULD + NP + lr];
-                    const double lbk = sm.Li[lr * LD16 + 4 * kk + lq];
-                    uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lbk, uacc, 0, 0, 0);
+                    const double lb = sm.Li[lr * LD16 + 4 * kk + lq];
+                    uacc = __builtin_amdgcn_mfma_f64_16x16x4f64(a12, lb, uacc, 0, 0, 0);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
@@ -419,149 +525,110 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
                     sm.B[(k0 + lr) * LD16 + 4 * r + lq] = uacc[r];
                 }
             }
-        }
-        __syncthreads();
-
-        // ---- S6: trailing update with MFMA ----
-        // negated L21 fragments of this wave's block rows: lane l holds
-        // -L(16I + l%16, 4kk + l/16)
-        double lf[NTW][4];
+            __syncthreads();                                      // S5
+            // S6: B block rows below the panel
+            {
+                double uf[4];
 #pragma unroll
-        for (int i = 0; i < NTW; ++i) {
-            const int I = 2 * i + wr;
+                for (int kk = 0; kk < 4; ++kk) uf[kk] = sm.U[(4 * kk + lq) * ULD + NP + lr];
+                for (int I = P + 1; I < NT; ++I) {
+                    double4_t cb;
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk)
-                lf[i][kk] = (I < NT && I > P) ? -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq] : 0.0;
-        }
+                    for (int r = 0; r < 4; ++r) cb[r] = sm.B[(16 * I + lr) * LD16 + 4 * r + lq];
 #pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-            const int J = 2 * j + wc;
-            if (J >= NT || J <= P) continue;
-            double uf[4];
+                    for (int kk = 0; kk < 4; ++kk) {
+                        const double lfr = -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq];
+                        cb = __builtin_amdgcn_mfma_f64_16x16x4f64(uf[kk], lfr, cb, 0, 0, 0);
+                    }
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) uf[kk] = sm.U[(4 * kk + lq) * ULD + 16 * J + lr];
-#pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                const int I = 2 * i + wr;
-                if (I >= NT || I <= P) continue;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(uf[kk], lf[i][kk], acc[i][j], 0, 0, 0);
-            }
-        }
-        if constexpr (WITH_B) {
-            // B block rows below the panel: wave (I % 4) updates block row I.
-            double uf[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) uf[kk] = sm.U[(4 * kk + lq) * ULD + NP + lr];
-            for (int I = P + 1; I < NT; ++I) {
-                if ((I % kTileWaves) != w) continue;
-                double4_t cb;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) cb[r] = sm.B[(16 * I + lr) * LD16 + 4 * r + lq];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    const double lfr = -sm.L[(16 * (I - P) + lr) * LD16 + 4 * kk + lq];
-                    cb = __builtin_amdgcn_mfma_f64_16x16x4f64(uf[kk], lfr, cb, 0, 0, 0);
+                    for (int r = 0; r < 4; ++r) sm.B[(16 * I + lr) * LD16 + 4 * r + lq] = cb[r];
                 }
-#pragma unroll
-                for (int r = 0; r < 4; ++r) sm.B[(16 * I + lr) * LD16 + 4 * r + lq] = cb[r];
             }
-        }
-        __syncthreads();
-    }
-
-    // ---- write the factors ----
-#pragma unroll
-    for (int i = 0; i < NTW; ++i) {
-        const int I = 2 * i + wr;
-        const int row = 16 * I + lr;
-#pragma unroll
-        for (int j = 0; j < NTW; ++j) {
-            const int J = 2 * j + wc;
-            if (I >= NT || J >= NT) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int col = 16 * J + 4 * r + lq;
-                if (row < n && col < n) A[row + (size_t)col * ldda] = acc[i][j][r];
-            }
+            __syncthreads();                                      // S6
+        } else {
+            __syncthreads();                                      // S3
+            __syncthreads();                                      // S4a
+            __syncthreads();                                      // S4b
+            __syncthreads();                                      // S5
+            __syncthreads();                                      // S6
         }
     }
-    if (t == 0) info_array[mat] = sm.info;
 
     if constexpr (WITH_B) {
-        // ---- back substitution U X = Y, block rows bottom-up ----
-        double* T = sm.U;                         // [5][16][LD16]: 4 partials + U_II
+        double* T = sm.U;
         const int nb_real = (n + 15) / 16;
         for (int I = nb_real - 1; I >= 0; --I) {
-            double4_t part = {0.0, 0.0, 0.0, 0.0};
-            bool any = false;
+            __syncthreads();                                      // partials ready
+            // lane 4c + q: right-hand side c, rows q, q+4, q+8, q+12 of the
+            // block; the solved entry of row k is broadcast within the quad
+            const int c = l >> 2, q = l & 3;
+            double y[4], rd[4];
 #pragma unroll
-            for (int i = 0; i < NTW; ++i) {
-                if (2 * i + wr != I) continue;
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int i = q + 4 * s4;
+                double v = sm.B[(16 * I + i) * LD16 + c];
 #pragma unroll
-                for (int j = 0; j < NTW; ++j) {
-                    const int J = 2 * j + wc;
-                    if (J >= NT || J <= I) continue;
-                    any = true;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const double xb = sm.B[(16 * J + 4 * r + lq) * LD16 + lr];
-                        part = __builtin_amdgcn_mfma_f64_16x16x4f64(acc[i][j][r], xb, part, 0, 0, 0);
-                    }
-                }
-                // diagonal tile to the solve scratch
-#pragma unroll
-                for (int j = 0; j < NTW; ++j) {
-                    if (2 * j + wc != I) continue;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) T[(4 * 16 + lr) * LD16 + 4 * r + lq] = acc[i][j][r];
-                }
+                for (int qq = 0; qq < kTileOwners; ++qq) v -= T[(qq * 16 + i) * LD16 + c];
+                y[s4] = v;
+                rd[s4] = 1.0 / T[(48 + i) * LD16 + i];
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) T[(w * 16 + 4 * r + lq) * LD16 + lr] = any ? part[r] : 0.0;
-            __syncthreads();
-            if (w == 0) {
-                // lane 4c + q: right-hand side c, rows q, q+4, q+8, q+12 of the
-                // block; the solved entry of row k is broadcast within the quad
-                const int c = l >> 2, q = l & 3;
-                double y[4], rd[4];
+            for (int k = 15; k >= 0; --k) {
+                const int sk = k >> 2;
+                y[sk] = (q == (k & 3)) ? y[sk] * rd[sk] : y[sk];
+                double xk;
+                switch (k & 3) {
+                    case 0: xk = quad_bcast_d<0>(y[sk]); break;
+                    case 1: xk = quad_bcast_d<1>(y[sk]); break;
+                    case 2: xk = quad_bcast_d<2>(y[sk]); break;
+                    default: xk = quad_bcast_d<3>(y[sk]); break;
+                }
 #pragma unroll
                 for (int s4 = 0; s4 < 4; ++s4) {
                     const int i = q + 4 * s4;
-                    double v = sm.B[(16 * I + i) * LD16 + c];
-#pragma unroll
-                    for (int qq = 0; qq < kTileWaves; ++qq) v -= T[(qq * 16 + i) * LD16 + c];
-                    y[s4] = v;
-                    rd[s4] = 1.0 / T[(64 + i) * LD16 + i];
+                    const double coef = (i < k) ? T[(48 + i) * LD16 + k] : 0.0;
+                    y[s4] = fma(-coef, xk, y[s4]);
                 }
-#pragma unroll
-                for (int k = 15; k >= 0; --k) {
-                    const int sk = k >> 2;
-                    y[sk] = (q == (k & 3)) ? y[sk] * rd[sk] : y[sk];
-                    double xk;
-                    switch (k & 3) {
-                        case 0: xk = quad_bcast_d<0>(y[sk]); break;
-                        case 1: xk = quad_bcast_d<1>(y[sk]); break;
-                        case 2: xk = quad_bcast_d<2>(y[sk]); break;
-                        default: xk = quad_bcast_d<3>(y[sk]); break;
-                    }
-#pragma unroll
-                    for (int s4 = 0; s4 < 4; ++s4) {
-                        const int i = q + 4 * s4;
-                        const double coef = (i < k) ? T[(64 + i) * LD16 + k] : 0.0;
-                        y[s4] = fma(-coef, xk, y[s4]);
-                    }
-                }
-#pragma unroll
-                for (int s4 = 0; s4 < 4; ++s4) sm.B[(16 * I + q + 4 * s4) * LD16 + c] = y[s4];
             }
-            __syncthreads();
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) sm.B[(16 * I + q + 4 * s4) * LD16 + c] = y[s4];
+            __syncthreads();                                      // block row solved
         }
-        double* B = dB_array[mat];
-        for (int idx = t; idx < n * nrhs; idx += kTileThreads) {
-            const int row = idx % n, c = idx / n;
-            B[row + (size_t)c * lddb] = sm.B[row * LD16 + c];
+    }
+}
+
+template <int NT, bool WITH_B>
+__global__ void __launch_bounds__(kTileThreads, 2)
+getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict__ ipiv_array,
+                  int* __restrict__ info_array, double** __restrict__ dB_array)
+{
+    constexpr int LD16 = TileSmem<NT, WITH_B>::LD16;
+    __shared__ TileSmem<NT, WITH_B> sm;
+    const int t = threadIdx.x;
+    const int w = t / kWave;
+    const int l = t % kWave;
+    const int mat = ta.map ? ta.map[blockIdx.x] : blockIdx.x;
+    TileCtx<NT, WITH_B> cx{sm,
+                           ta.n_arr ? ta.n_arr[mat] : ta.n,
+                           ta.nrhs_arr ? ta.nrhs_arr[mat] : ta.nrhs,
+                           ta.ldda_arr ? ta.ldda_arr[mat] : ta.ldda,
+                           ta.lddb_arr ? ta.lddb_arr[mat] : ta.lddb,
+                           dA_array[mat], ipiv_array[mat], WITH_B ? dB_array[mat] : nullptr,
+                           l, l % 16, l / 16};
+    if (cx.n <= 0) return;                // empty system: nothing to factor
+    switch (w) {
+        case 0: tile_wave_main<NT, WITH_B, 0>(cx); break;
+        case 1: tile_wave_main<NT, WITH_B, 1>(cx); break;
+        case 2: tile_wave_main<NT, WITH_B, 2>(cx); break;
+        default: panel_wave_main<NT, WITH_B>(cx); break;
+    }
+    __syncthreads();                      // sm.info final
+    if (t == 0) info_array[mat] = sm.info;
+    if constexpr (WITH_B) {
+        double* B = cx.B;
+        for (int idx = t; idx < cx.n * cx.nrhs; idx += kTileThreads) {
+            const int row = idx % cx.n, c = idx / cx.n;
+            B[row + (size_t)c * cx.lddb] = sm.B[row * LD16 + c];
         }
     }
 }
