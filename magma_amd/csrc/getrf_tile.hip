@@ -80,8 +80,8 @@ struct TileSmem {
     static constexpr int LD16 = 17;                 // padded row length of 16-wide buffers
     static constexpr int UCOLS = NP + (WITH_B ? 16 : 0);
     static constexpr int ULD = UCOLS + 1;
-    static constexpr int SLD = NP + 1;
-    static constexpr int S_DOUBLES = 32 * SLD;
+    static constexpr int SLD = NP + (WITH_B ? 16 : 0) + 1;   // staged rows: A's columns, then B's
+    static constexpr int S_DOUBLES = 33 * SLD;                    // 32 staged rows + a trash row
     static constexpr int U_DOUBLES = 16 * ULD;
     static constexpr int T_DOUBLES = (kTileWaves + 1) * 16 * LD16;
     static constexpr int UNION_DOUBLES =
@@ -107,6 +107,101 @@ struct TileMap {
     __device__ static constexpr int row(int w, int k) { return (kTileOwners * k + w) / NT; }
     __device__ static constexpr int col(int w, int k) { return (kTileOwners * k + w) % NT; }
 };
+
+// One 16-column panel on one wave (LAPACK dgetf2 semantics, lazy pivoting):
+// lane l holds the panel rows l (x0) and, with HAS1, l + 64 (x1); pos0/pos1
+// are their current LAPACK positions and alive0/alive1 whether they are still
+// candidates. All 16 columns are processed: in the last panel of a matrix
+// whose order is not a multiple of 16 the extra columns belong to the
+// identity padding, whose pivot is the padding row itself (so nothing
+// changes); only the ipiv and info writes are limited to real columns.
+//
+// The loop is software pipelined: step c updates column c+1 first, then
+// runs the pivot search of column c+1 (DPP maximum, ballots) in the same
+// branch-free block as the remaining rank-1 updates, so the reduction's
+// latency is hidden behind the FMAs.
+template <bool HAS1>
+__device__ __forceinline__ void panel_columns(double (&x0)[16], double (&x1)[16], int& pos0, int& pos1,
+                                              bool& alive0, bool& alive1, int& info_local, int* __restrict__ ipiv,
+                                              int k0, int n, int l)
+{
+    // idamax candidates of column c: NaN never wins, except at the first
+    // candidate position c, which keeps the pivot (counted as +inf)
+    double v0, v1 = -1.0, m;
+    unsigned long long b0, b1 = 0ull;
+    auto search = [&](int c) {
+        v0 = alive0 ? fabs(x0[c]) : -1.0;
+        if (pos0 == c && v0 != v0) v0 = INFINITY;
+        if (HAS1) {
+            v1 = alive1 ? fabs(x1[c]) : -1.0;
+            if (pos1 == c && v1 != v1) v1 = INFINITY;
+        }
+        m = wave_max_dpp(HAS1 ? fmax(v0, v1) : v0);
+        b0 = __ballot(v0 == m);
+        if (HAS1) b1 = __ballot(v1 == m);
+    };
+    search(0);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        if (__popcll(b0) + __popcll(b1) != 1) {
+            // exact tie: LAPACK keeps the lowest current position
+            const int cand = min(v0 == m ? pos0 : INT_MAX, (HAS1 && v1 == m) ? pos1 : INT_MAX);
+            const int mpos = wave_min_dpp(cand);
+            b0 = __ballot(v0 == m && pos0 == mpos);
+            if (HAS1) b1 = __ballot(v1 == m && pos1 == mpos);
+        }
+        const bool in1 = HAS1 && b0 == 0ull;
+        const int pl = in1 ? first_lane(b1) : first_lane(b0);
+        // pivot and next-column entry of the pivot row (scalar registers)
+        const double pv = readlane_d(in1 ? x1[c] : x0[c], pl);
+        const int cn = c + 1 < 16 ? c + 1 : c;      // next column (unused at c = 15)
+        const double un = readlane_d(in1 ? x1[cn] : x0[cn], pl);
+        const int pos_p = __builtin_amdgcn_readlane(in1 ? pos1 : pos0, pl);
+        if (l == 0 && k0 + c < n) ipiv[k0 + c] = k0 + pos_p + 1;
+        // LAPACK interchange of positions c and pos_p
+        const bool me0 = !in1 && l == pl, me1 = in1 && l == pl;
+        pos0 = me0 ? c : (pos0 == c ? pos_p : pos0);
+        alive0 = alive0 && !me0;
+        if (HAS1) {
+            pos1 = me1 ? c : (pos1 == c ? pos_p : pos1);
+            alive1 = alive1 && !me1;
+        }
+        // multipliers: reciprocal scaling, or division when |pivot| < sfmin
+        const double rinv = 1.0 / pv;
+        double l0 = x0[c] * rinv, l1 = HAS1 ? x1[c] * rinv : 0.0;
+        if (!(pv != 0.0 && fabs(pv) >= kSfminD)) {
+            if (pv != 0.0) {
+                l0 = x0[c] / pv;
+                if (HAS1) l1 = x1[c] / pv;
+            } else {
+                l0 = x0[c];
+                l1 = HAS1 ? x1[c] : 0.0;
+                if (info_local == 0 && k0 + c < n) info_local = k0 + c + 1;
+            }
+        }
+        // rows still alive take the multiplier; others stay as they are
+        const double g0 = alive0 ? l0 : 0.0;
+        x0[c] = alive0 ? l0 : x0[c];
+        double g1 = 0.0;
+        if (HAS1) {
+            g1 = alive1 ? l1 : 0.0;
+            x1[c] = alive1 ? l1 : x1[c];
+        }
+        if (c + 1 < 16) {
+            x0[cn] = fma(-g0, un, x0[cn]);
+            if (HAS1) x1[cn] = fma(-g1, un, x1[cn]);
+            search(cn);
+            // the rest of the pivot row and the remaining rank-1 updates
+#pragma unroll
+            for (int c2 = 0; c2 < 16; ++c2) {
+                if (c2 <= cn) continue;
+                const double u = readlane_d(in1 ? x1[c2] : x0[c2], pl);
+                x0[c2] = fma(-g0, u, x0[c2]);
+                if (HAS1) x1[c2] = fma(-g1, u, x1[c2]);
+            }
+        }
+    }
+}
 
 // Barrier schedule shared by the two wave programs below (each __syncthreads
 // in one program has a partner in the other, in the same order):
@@ -179,38 +274,59 @@ __device__ __forceinline__ void tile_wave_main(const TileCtx<NT, WITH_B>& cx)
         __syncthreads();                                          // S1
         __syncthreads();                                          // S2 (panel wave)
 
-        // S3: stage the rows that move; reload the factored panel
+        // S3: stage the rows that move; reload the factored panel. A row
+        // that stays is written to the trash row, so no lane branches; each
+        // row block's slot is read once.
+        constexpr int kTrash = 32;
+        int slot_of_block[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+            const int s = (I >= P) ? sm.slotof[16 * (I - P) + lr] : -1;
+            slot_of_block[I] = s >= 0 ? s : kTrash;
+        }
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
             if (!TileMap<NT>::valid(w, k)) continue;
             const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
             if (I < P) continue;
-            const int rr = 16 * (I - P) + lr;
             if (J == P) {
+                const int rr = 16 * (I - P) + lr;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) acc[k][r] = sm.L[rr * LD16 + 4 * r + lq];
             } else {
-                const int slot = sm.slotof[rr];
-                if (slot >= 0) {
+                const int base = slot_of_block[I] * SLD + 16 * J + lq;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) sm.U[slot * SLD + 16 * J + 4 * r + lq] = acc[k][r];
-                }
+                for (int r = 0; r < 4; ++r) sm.U[base + 4 * r] = acc[k][r];
             }
         }
         __syncthreads();                                          // S3
 
-        // S4a: apply the row permutation from the staging area
+        // S4a: apply the row permutation from the staging area (rows that
+        // stay read the trash row and keep their value)
+        bool moved_of_block[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+            bool mv = false;
+            int s = kTrash;
+            if (I >= P) {
+                const int rr = 16 * (I - P) + lr;
+                const int src = sm.rowmap[rr];
+                mv = src != rr;
+                if (mv) s = sm.slotof[src];
+            }
+            moved_of_block[I] = mv;
+            slot_of_block[I] = s;
+        }
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
             if (!TileMap<NT>::valid(w, k)) continue;
             const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
             if (I < P || J == P) continue;
-            const int rr = 16 * (I - P) + lr;
-            const int src = sm.rowmap[rr];
-            if (src != rr) {
-                const int slot = sm.slotof[src];
+            const int base = slot_of_block[I] * SLD + 16 * J + lq;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[k][r] = sm.U[slot * SLD + 16 * J + 4 * r + lq];
+            for (int r = 0; r < 4; ++r) {
+                const double v = sm.U[base + 4 * r];
+                acc[k][r] = moved_of_block[I] ? v : acc[k][r];
             }
         }
         __syncthreads();                                          // S4a
@@ -362,77 +478,8 @@ __device__ __forceinline__ void panel_wave_main(const TileCtx<NT, WITH_B>& cx)
             int pos0 = l, pos1 = l + 64;
             bool alive0 = l < mp, alive1 = has1 && (l + 64 < mp);
             int info_local = 0;
-            const int creal = min(16, n - k0);        // real columns in this panel
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                // constant trip count (the body holds cross-lane operations,
-                // which rule out unrolling with an early exit)
-                if (c >= creal) continue;
-                // idamax: NaN never wins, except at the first candidate
-                // position c, which keeps the pivot (counted as +inf)
-                double v0 = alive0 ? fabs(x0[c]) : -1.0;
-                double v1 = alive1 ? fabs(x1[c]) : -1.0;
-                if (pos0 == c && v0 != v0) v0 = INFINITY;
-                if (pos1 == c && v1 != v1) v1 = INFINITY;
-                const double m = wave_max_dpp(fmax(v0, v1));
-                unsigned long long b0 = __ballot(v0 == m);
-                unsigned long long b1 = __ballot(v1 == m);
-                if (__popcll(b0) + __popcll(b1) != 1) {
-                    // exact tie: LAPACK keeps the lowest current position
-                    const int cand = min(v0 == m ? pos0 : INT_MAX, v1 == m ? pos1 : INT_MAX);
-                    const int mpos = wave_min_dpp(cand);
-                    b0 = __ballot(v0 == m && pos0 == mpos);
-                    b1 = __ballot(v1 == m && pos1 == mpos);
-                }
-                const int ps = (b0 != 0ull) ? 0 : 1;
-                const int pl = (b0 != 0ull) ? first_lane(b0) : first_lane(b1);
-                // the pivot row, read out of lane pl into scalar registers
-                double u[16];
-                if (ps == 0) {
-#pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2)
-                        if (c2 >= c) u[c2] = readlane_d(x0[c2], pl);
-                } else {
-#pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2)
-                        if (c2 >= c) u[c2] = readlane_d(x1[c2], pl);
-                }
-                const int pos_p = __builtin_amdgcn_readlane(ps ? pos1 : pos0, pl);
-                if (l == 0) ipiv[k0 + c] = k0 + pos_p + 1;
-                // LAPACK interchange of positions c and pos_p
-                const bool me0 = (ps == 0 && l == pl), me1 = (ps == 1 && l == pl);
-                pos0 = me0 ? c : (pos0 == c ? pos_p : pos0);
-                pos1 = me1 ? c : (pos1 == c ? pos_p : pos1);
-                alive0 = alive0 && !me0;
-                alive1 = alive1 && !me1;
-                const double pv = u[c];
-                double l0 = x0[c], l1 = x1[c];
-                if (pv != 0.0) {
-                    if (fabs(pv) >= kSfminD) {
-                        const double rinv = 1.0 / pv;
-                        l0 *= rinv;
-                        l1 *= rinv;
-                    } else {
-                        l0 /= pv;
-                        l1 /= pv;
-                    }
-                } else if (info_local == 0) {
-                    info_local = k0 + c + 1;
-                }
-                // rows still alive take the multiplier; others stay as they are
-                x0[c] = alive0 ? l0 : x0[c];
-                const double f0 = alive0 ? l0 : 0.0;
-#pragma unroll
-                for (int c2 = 0; c2 < 16; ++c2)
-                    if (c2 > c) x0[c2] = fma(-f0, u[c2], x0[c2]);
-                if (has1) {
-                    x1[c] = alive1 ? l1 : x1[c];
-                    const double f1 = alive1 ? l1 : 0.0;
-#pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2)
-                        if (c2 > c) x1[c2] = fma(-f1, u[c2], x1[c2]);
-                }
-            }
+            if (has1) panel_columns<true>(x0, x1, pos0, pos1, alive0, alive1, info_local, ipiv, k0, n, l);
+            else panel_columns<false>(x0, x1, pos0, pos1, alive0, alive1, info_local, ipiv, k0, n, l);
             // every panel row (padding included) to its LAPACK position
             if (l < mp) {
                 sm.rowmap[pos0] = l;
@@ -485,22 +532,37 @@ __device__ __forceinline__ void panel_wave_main(const TileCtx<NT, WITH_B>& cx)
         __syncthreads();                                          // S2
 
         if constexpr (WITH_B) {
-            // S3: gather B's rows in their new order
-            constexpr int BQ = (NP * 16 + kWave - 1) / kWave;
-            double btmp[BQ];
+            // S3: stage B's moved rows beside A's (staging columns NP..NP+15);
+            // lane l copies its rows l and l + 64, a row that stays goes to
+            // the trash row
+            constexpr int SLD = TileSmem<NT, WITH_B>::SLD;
 #pragma unroll
-            for (int q = 0; q < BQ; ++q) {
-                const int idx = l + q * kWave;
-                const int i = idx / 16, c = idx % 16;
-                if (i < mp) btmp[q] = sm.B[(k0 + sm.rowmap[i]) * LD16 + c];
+            for (int h = 0; h < 2; ++h) {
+                const int i = l + 64 * h;
+                if (h * 64 >= NP) continue;
+                int s = 32;
+                if (i < mp) {
+                    const int sl = sm.slotof[i];
+                    s = sl >= 0 ? sl : 32;
+                }
+                const double* src = &sm.B[(k0 + (i < mp ? i : 0)) * LD16];
+                double* dst = &sm.U[s * SLD + NP];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) dst[c] = src[c];
             }
             __syncthreads();                                      // S3
-            // S4a
+            // S4a: B's rows into their new positions
 #pragma unroll
-            for (int q = 0; q < BQ; ++q) {
-                const int idx = l + q * kWave;
-                const int i = idx / 16, c = idx % 16;
-                if (i < mp) sm.B[(k0 + i) * LD16 + c] = btmp[q];
+            for (int h = 0; h < 2; ++h) {
+                const int i = l + 64 * h;
+                if (h * 64 >= NP || i >= mp) continue;
+                const int src = sm.rowmap[i];
+                if (src != i) {
+                    const double* from = &sm.U[sm.slotof[src] * SLD + NP];
+                    double* to = &sm.B[(k0 + i) * LD16];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) to[c] = from[c];
+                }
             }
             __syncthreads();                                      // S4a
             // S4b: B's panel rows beside U12

@@ -106,6 +106,38 @@ __device__ __forceinline__ double wave_max_dpp(double v)
     return readlane_d(v, 63);
 }
 
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK, 0xF,
+                                                      false));
+}
+
+// Maximum of a float over the 64 lanes (same pattern as wave_max_dpp; each
+// stage can issue as a single DPP-sourced v_max_f32).
+__device__ __forceinline__ float wave_max_dpp_f(float v)
+{
+    v = fmaxf(v, dpp_f<kDppQuadXor1>(v));
+    v = fmaxf(v, dpp_f<kDppQuadXor2>(v));
+    v = fmaxf(v, dpp_f<kDppHalfMirror>(v));
+    v = fmaxf(v, dpp_f<kDppMirror>(v));
+    v = fmaxf(v, dpp_f<kDppBcast15, 0xA>(v));
+    v = fmaxf(v, dpp_f<kDppBcast31, 0xC>(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Maximum of an int over the 64 lanes (each stage a DPP-sourced v_max_i32).
+__device__ __forceinline__ int wave_max_dpp_i(int v)
+{
+    v = max(v, dpp_i<kDppQuadXor1>(v));
+    v = max(v, dpp_i<kDppQuadXor2>(v));
+    v = max(v, dpp_i<kDppHalfMirror>(v));
+    v = max(v, dpp_i<kDppMirror>(v));
+    v = max(v, dpp_i<kDppBcast15, 0xA>(v));
+    v = max(v, dpp_i<kDppBcast31, 0xC>(v));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ int wave_min_dpp(int v)
 {
     v = min(v, dpp_i<kDppQuadXor1>(v));
