@@ -75,6 +75,8 @@ def parse():
                     help="also time BF16-LU + FP64 refinement against FP64 dgesv on 2,000 x 256^2 (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="also time the differentiable solve (forward + backward) on 8,192 x 64^2 (0 = skip)")
+    ap.add_argument("--vbatched", type=int, default=1,
+                    help="also run the variable-size leg (50,000 systems, n ~ U[8,512], nrhs ~ U[1,16]); 0 skips")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this many MiB once on the device (a kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -224,6 +226,74 @@ def adjoint_leg(torch, dev, seed, steps=5, batch=8192, n=64, nrhs=8):
     return res
 
 
+def vbatched_leg(torch, mb, dev, seed, steps=3, batch=50000, nmin=8, nmax=512, rmax=16, spot=48):
+    """Variable-size solve: n_i ~ U[nmin, nmax], nrhs_i ~ U[1, rmax], bucketed
+    by the plan with the autotuned variant per bucket, captured once as a
+    hipGraph and replayed. Inputs are restored from pristine copies before each
+    replay, outside the timed region (events bracket only the replay)."""
+    import numpy as np
+    from magma_amd import autotune
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, batch).astype(np.int32)
+    rs = rng.integers(1, rmax + 1, batch).astype(np.int32)
+    t_tune = time.perf_counter()
+    variants = autotune.load_variants(ns, rs)
+    t_tune = time.perf_counter() - t_tune
+    A = mb.VBatch(ns, ns, device=dev)
+    B = mb.VBatch(ns, rs, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A.data.uniform_(-1.0, 1.0, generator=gen)
+    B.data.uniform_(-1.0, 1.0, generator=gen)
+    ipiv = mb.IpivV(ns, dev)
+    info = torch.zeros(batch, dtype=torch.int32, device=dev)
+    A0, B0 = A.data.clone(), B.data.clone()
+    plan = mb.VbatchedPlan(ns, rs, variants=variants)
+    plan.capture(A, B, ipiv, info)
+    times = []
+    for k in range(steps + 1):                        # replay 0 is warm-up
+        A.data.copy_(A0)
+        B.data.copy_(B0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        plan.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1) / 1e3)
+    t = sum(times) / len(times)
+    n64, r64 = ns.astype(np.float64), rs.astype(np.float64)
+    flop = float(sum(flops_getrf(x) + flops_getrs(x, r) for x, r in zip(n64, r64)))
+    # bytes: A read + LU written, B read + X written, ipiv and info written
+    nbytes = float((16 * (n64 * n64 + n64 * r64) + 4 * n64 + 4).sum())
+    # backward error on a spot sample, FP64 numpy
+    worst = 0.0
+    for i in rng.choice(batch, size=min(spot, batch), replace=False):
+        m, r = int(ns[i]), int(rs[i])
+        a0, b0 = int(A.offs[i]), int(B.offs[i])
+        Ai = A0[a0:a0 + m * m].view(m, m).T.cpu().numpy()
+        Bi = B0[b0:b0 + m * r].view(r, m).T.cpu().numpy()
+        Xi = B.get(int(i))
+        rinf = np.abs(Bi - Ai @ Xi).sum(1).max()
+        worst = max(worst, rinf / (np.abs(Ai).sum(1).max() * np.abs(Xi).sum(1).max()))
+    buckets = [{"id": b["id"], "variant": autotune.VARIANT_NAMES.get(b["variant"], str(b["variant"])),
+                "systems": b["count"], "max_n": b["max_n"]} for b in plan.buckets()]
+    n_info = int((info != 0).sum().item())
+    plan.close()
+    del A, B, A0, B0, ipiv, info
+    torch.cuda.empty_cache()
+    return {"workload": f"dgesv_vbatched {batch} systems, n ~ U[{nmin},{nmax}], nrhs ~ U[1,{rmax}], FP64, "
+                        "one hipGraph replay per step",
+            "steps": steps, "ms_per_replay": round(t * 1e3, 3), "solves_per_s": round(batch / t, 1),
+            "fp64_tflops": round(flop / t / 1e12, 3), "fp64_frac": round(flop / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+            "hbm_gbs": round(nbytes / t / 1e9, 1), "hbm_frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+            "floor_ms": {"fp64_mfma": round(flop / (FP64_PEAK_TFLOPS * 1e12) * 1e3, 3),
+                         "hbm": round(nbytes / (HBM_PEAK_GBS * 1e9) * 1e3, 3)},
+            "backward_error_max_spot": float(worst), "spot_systems": int(min(spot, batch)),
+            "nonzero_info": n_info, "tuning_s": round(t_tune, 2), "buckets": buckets}
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -370,6 +440,11 @@ def main():
         print("[bench] adjoint leg", file=sys.stderr, flush=True)
         adjoint = adjoint_leg(torch, dev, args.seed)
 
+    vb = None
+    if args.vbatched:
+        print("[bench] variable-size leg", file=sys.stderr, flush=True)
+        vb = vbatched_leg(torch, mb, dev, args.seed)
+
     cpu = None
     if A_host is not None:
         print(f"[bench] cpu baseline on {A_host.shape[0]} systems", file=sys.stderr, flush=True)
@@ -398,6 +473,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
         "adjoint": adjoint,
+        "vbatched": vb,
     }
     print(json.dumps(out), flush=True)
 
