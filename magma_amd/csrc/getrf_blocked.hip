@@ -165,40 +165,76 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
 
 // ---------------------------------------------------------------------------
 // 2. Row interchanges of panel k..k+jb-1 on every column outside the panel.
+//    The jb interchanges touch at most 2*jb rows: the panel rows k..k+jb-1
+//    and the pivot rows below them. Lane i of each wave takes one touched
+//    destination row and traces it back through the interchanges (last to
+//    first) to the row whose content ends there; the wave then moves one
+//    column at a time, every lane loading its source before any lane stores
+//    (the composition equals applying the interchanges in order, as dlaswp).
+//    Interchanges outside [k+j, m) cannot come from a valid panel and are
+//    treated as none.
 // ---------------------------------------------------------------------------
+constexpr int kLaswpCols = 32;     // columns per workgroup
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __restrict__ ipiv_array)
 {
+    __shared__ int s_piv[32];
     const int b = sz.mat(blockIdx.y);
     int m, n, ld;
     sz.get(b, m, n, ld);
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(nb, minmn - k);
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int other = n - jb;                       // columns outside the panel
-    if (c >= other) return;
-    const int col = c < k ? c : c + jb;
-    T* A = dA_array[b] + (size_t)col * ld;
-    const int* ipiv = ipiv_array[b];
-    for (int j = k; j < k + jb; ++j) {
-        const int p = ipiv[j] - 1;
-        if (p > j && p < m) {
-            T tmp = A[j];
-            A[j] = A[p];
-            A[p] = tmp;
-        }
+    const int cbeg = blockIdx.x * kLaswpCols;
+    if (cbeg >= other) return;
+    const int t = threadIdx.x;
+    if (t < jb) {
+        const int p = ipiv_array[b][k + t] - 1;
+        s_piv[t] = (p > k + t && p < m) ? p : k + t;
+    }
+    __syncthreads();
+    const int l = t % kWave, w = t / kWave;
+    int dst = -1;
+    if (l < jb) {
+        dst = k + l;
+    } else if (l < 2 * jb) {
+        dst = s_piv[l - jb];
+        if (dst < k + jb) dst = -1;                 // a panel row: already a lane of its own
+    }
+    int src = dst;
+    for (int j = jb - 1; j >= 0; --j) {
+        const int p = s_piv[j];
+        src = (src == k + j) ? p : (src == p ? k + j : src);
+    }
+    const bool move = dst >= 0 && src != dst;
+    const int cend = min(other, cbeg + kLaswpCols);
+    for (int c = cbeg + w; c < cend; c += 256 / kWave) {
+        const int col = c < k ? c : c + jb;
+        T* A = dA_array[b] + (size_t)col * ld;
+        T v = T(0);
+        if (move) v = A[src];
+        __builtin_amdgcn_wave_barrier();            // all sources read before any store
+        if (move) A[dst] = v;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 // ---------------------------------------------------------------------------
-// 3. A12 := L11^{-1} A12, one thread per column of A12.
+// 3. A12 := L11^{-1} A12 (unit lower). NB lanes run down the rows of one
+//    column (64/NB columns per wave instruction), so loads and stores are
+//    contiguous; the solved x_j reaches the rows below through a cross-lane
+//    read. Same operation order as a column-oriented dtrsm.
 // ---------------------------------------------------------------------------
+constexpr int kTrsmCols = 64;      // columns per workgroup
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
 trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array)
 {
+    constexpr int CPW = kWave / NB;                 // columns per wave pass
     __shared__ T sL[NB * NB];
     const int b = sz.mat(blockIdx.y);
     int m, n, ld;
@@ -208,29 +244,30 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array)
     const int jb = min(NB, minmn - k);
     const int ncols = n - k - jb;
     if (ncols <= 0) return;
-    const int c0 = blockIdx.x * blockDim.x;
+    const int c0 = blockIdx.x * kTrsmCols;
     if (c0 >= ncols) return;
+    const int t = threadIdx.x;
     const T* L = dA_array[b] + k + (size_t)k * ld;
-    for (int idx = threadIdx.x; idx < jb * jb; idx += blockDim.x) {
-        const int i = idx % jb, j = idx / jb;
-        sL[i + j * NB] = L[i + (size_t)j * ld];
+    for (int idx = t; idx < NB * NB; idx += 256) {
+        const int i = idx % NB, j = idx / NB;
+        sL[idx] = (i < jb && j < jb) ? L[i + (size_t)j * ld] : T(0);
     }
     __syncthreads();
-    const int c = c0 + threadIdx.x;
-    if (c >= ncols) return;
-    T* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
-    T x[NB];
+    const int l = t % kWave, w = t / kWave;
+    const int i = l % NB, cq = l / NB;
+    const int cend = min(ncols, c0 + kTrsmCols);
+    for (int cb = c0 + w * CPW; cb < cend; cb += (256 / kWave) * CPW) {
+        const int c = cb + cq;
+        const bool on = c < cend && i < jb;
+        T* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
+        T x = on ? X[i] : T(0);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) x[i] = i < jb ? X[i] : T(0);
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-            if (i > j && i < jb) x[i] = fma(-sL[i + j * NB], x[j], x[i]);
+        for (int j = 0; j < NB - 1; ++j) {
+            const T xj = shfl_t(x, cq * NB + j);
+            if (i > j) x = fma(-sL[i + j * NB], xj, x);
+        }
+        if (on) X[i] = x;
     }
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-        if (i < jb) X[i] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -447,12 +484,13 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
         }
         MGI_LAUNCH_CHECK();
         const int other = max_n;   // upper bound of the columns outside the panel
-        laswp_kernel<T><<<dim3(ceildiv(other, 256), batch), 256, 0, stream>>>(sz, k, NB, dA_array, ipiv_array);
+        laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
+                                                                                   ipiv_array);
         MGI_LAUNCH_CHECK();
         const int rest_n = max_n - k - 1;
         const int rest_m = max_m - k - 1;
         if (rest_n > 0) {
-            trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, 256), batch), 256, 0, stream>>>(sz, k, dA_array);
+            trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array);
             MGI_LAUNCH_CHECK();
             if (rest_m > 0) {
                 dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);

@@ -9,7 +9,11 @@
 //
 // Each step is column-oriented (an axpy of the solved entry into the rows not
 // yet solved, with fused multiply-add), the order reference BLAS dtrsm uses
-// for the NoTrans case.
+// for the NoTrans case. The NoTrans sweeps are blocked by 64 rows: one wave
+// per right-hand side solves the diagonal block with cross-lane reads (no
+// workgroup barrier inside it), then all threads apply the block's 64
+// columns to the remaining rows, reading the factor down its columns. The
+// per-row operation order is unchanged by the blocking.
 #include "magma_internal.h"
 #include "lu_device.h"
 #include "getrf_kernels.h"
@@ -38,18 +42,24 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     const int* ipiv = ipiv_array[b];
     T* B = dB_array[b] + (size_t)c0 * lddb;
 
-    for (int i = tid; i < n; i += kRsThreads) perm[i] = i;
+    // (P^T B)(i,:) = B(perm[i],:), P the product of the interchanges
+    // k <-> ipiv[k]-1 in order. Each row is traced back through the
+    // interchanges (last to first) to its source; an interchange outside
+    // [k, n) cannot come from a valid LU and is treated as none.
+    int* spiv = perm + sz.max_n;
+    for (int i = tid; i < n; i += kRsThreads) {
+        const int p = ipiv[i] - 1;
+        spiv[i] = (p > i && p < n) ? p : i;
+    }
     __syncthreads();
-    if (tid == 0) {
-        // Compose the LAPACK interchange sequence into a row map:
-        // (P^T B)(i,:) = B(perm[i],:). An interchange outside [k, n) cannot
-        // come from a valid LU; it is ignored rather than allowed to index
-        // outside the map.
-        for (int k = 0; k < n; ++k) {
-            int p = ipiv[k] - 1;
-            if (p < k || p >= n) continue;
-            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+    for (int r0 = 0; r0 < n; r0 += kRsThreads) {
+        const int r = r0 + tid;
+        int src = r;
+        for (int k = n - 1; k >= 0; --k) {
+            const int p = spiv[k];
+            src = (src == k) ? p : (src == p ? k : src);
         }
+        if (r < n) perm[r] = src;
     }
     __syncthreads();
 
@@ -59,20 +69,88 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
             sB[i + c * n] = B[perm[i] + (size_t)c * lddb];
         }
         __syncthreads();
-        for (int k = 0; k < n - 1; ++k) {                  // unit lower, forward
-            const int rows = n - k - 1;
-            for (int idx = tid; idx < rows * nr; idx += kRsThreads) {
-                int i = k + 1 + idx % rows, c = idx / rows;
-                sB[i + c * n] = fma(-A[i + (size_t)k * ldda], sB[k + c * n], sB[i + c * n]);
+        const int l = tid % kWave, w = tid / kWave;
+        constexpr int kWaves = kRsThreads / kWave;
+        for (int kb = 0; kb < n; kb += kWave) {            // unit lower, forward
+            const int nbk = min(kWave, n - kb);
+            const int r = kb + l;
+            if (w < nr) {
+                // lane l holds row r of the diagonal block (strictly lower part)
+                T lrow[kWave];
+                const T* Lc = A + r + (size_t)kb * ldda;
+#pragma unroll
+                for (int j = 0; j < kWave; ++j) lrow[j] = (l > j && l < nbk) ? Lc[(size_t)j * ldda] : T(0);
+                for (int c = w; c < nr; c += kWaves) {
+                    T x = l < nbk ? sB[r + c * n] : T(0);
+#pragma unroll
+                    for (int j = 0; j < kWave - 1; ++j) {
+                        const T xj = readlane_t(x, j);
+                        if (l > j) x = fma(-lrow[j], xj, x);
+                    }
+                    if (l < nbk) sB[r + c * n] = x;
+                }
+            }
+            __syncthreads();
+            // rows below exist only after a full block
+            for (int i = kb + kWave + tid; i < n; i += kRsThreads) {
+                T acc[kRsCols];
+#pragma unroll
+                for (int c = 0; c < kRsCols; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
+                const T* Li = A + i + (size_t)kb * ldda;
+#pragma unroll 16
+                for (int j = 0; j < kWave; ++j) {
+                    const T lij = Li[(size_t)j * ldda];
+#pragma unroll
+                    for (int c = 0; c < kRsCols; ++c)
+                        if (c < nr) acc[c] = fma(-lij, sB[kb + j + c * n], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < kRsCols; ++c)
+                    if (c < nr) sB[i + c * n] = acc[c];
             }
             __syncthreads();
         }
-        for (int k = n - 1; k >= 0; --k) {                 // non-unit upper, backward
-            if (tid < nr) sB[k + tid * n] = sB[k + tid * n] / A[k + (size_t)k * ldda];
+        for (int kend = n; kend > 0; kend -= kWave) {      // non-unit upper, backward
+            const int kb = max(0, kend - kWave);
+            const int nbk = kend - kb;
+            const int r = kb + l;
+            if (w < nr) {
+                // lane l holds row r of the diagonal block (strictly upper part)
+                T urow[kWave];
+                const T* Uc = A + r + (size_t)kb * ldda;
+#pragma unroll
+                for (int j = 0; j < kWave; ++j) urow[j] = (l < j && j < nbk) ? Uc[(size_t)j * ldda] : T(0);
+                const T ud = l < nbk ? A[r + (size_t)r * ldda] : T(1);
+                for (int c = w; c < nr; c += kWaves) {
+                    T x = l < nbk ? sB[r + c * n] : T(0);
+#pragma unroll
+                    for (int j = kWave - 1; j >= 0; --j) {
+                        if (j < nbk) {
+                            const T xj = readlane_t(x, j) / readlane_t(ud, j);
+                            if (l == j) x = xj;
+                            else if (l < j) x = fma(-urow[j], xj, x);
+                        }
+                    }
+                    if (l < nbk) sB[r + c * n] = x;
+                }
+            }
             __syncthreads();
-            for (int idx = tid; idx < k * nr; idx += kRsThreads) {
-                int i = idx % k, c = idx / k;
-                sB[i + c * n] = fma(-A[i + (size_t)k * ldda], sB[k + c * n], sB[i + c * n]);
+            // rows above exist only above a full block
+            for (int i = tid; i < kb; i += kRsThreads) {
+                T acc[kRsCols];
+#pragma unroll
+                for (int c = 0; c < kRsCols; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
+                const T* Ui = A + i + (size_t)kb * ldda;
+#pragma unroll 16
+                for (int j = kWave - 1; j >= 0; --j) {
+                    const T uij = Ui[(size_t)j * ldda];
+#pragma unroll
+                    for (int c = 0; c < kRsCols; ++c)
+                        if (c < nr) acc[c] = fma(-uij, sB[kb + j + c * n], acc[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < kRsCols; ++c)
+                    if (c < nr) sB[i + c * n] = acc[c];
             }
             __syncthreads();
         }
@@ -110,7 +188,7 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     }
 }
 
-size_t getrs_generic_bytes(int n, int elem) { return (size_t)n * kRsCols * elem + (size_t)n * sizeof(int); }
+size_t getrs_generic_bytes(int n, int elem) { return (size_t)n * kRsCols * elem + 2 * (size_t)n * sizeof(int); }
 size_t getrs_generic_bytes(int n) { return getrs_generic_bytes(n, sizeof(double)); }
 
 template <typename T>

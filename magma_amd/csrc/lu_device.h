@@ -36,6 +36,14 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
+__device__ __forceinline__ float readlane_t(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+__device__ __forceinline__ float shfl_t(float v, int src) { return __shfl(v, src, kWave); }
+__device__ __forceinline__ double shfl_t(double v, int src) { return shfl_d(v, src); }
+
 // Arg-max of |x| across the wave with LAPACK idamax tie breaking: the largest
 // value wins and, among equal values, the smallest index. `v` must be >= 0 for
 // real candidates and negative for lanes holding no candidate.
