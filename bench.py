@@ -149,9 +149,10 @@ def backward_errors(torch, A_cm, X_cm, B_cm):
 
 
 def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n=256, nrhs=1, cond=1e2):
-    """BF16-update LU + FP64 refinement (FGMRES) against FP64 dgesv_batched on
-    the same batch of svd_geo systems: solves/s, FP64-credited GFLOP/s,
-    refinement steps and the worst backward / forward errors."""
+    """BF16- and FP16-update LU + FP64 refinement (classical and FGMRES)
+    against FP64 dgesv_batched on the same batch of svd_geo systems: solves/s,
+    FP64-credited GFLOP/s, refinement steps and the worst backward / forward
+    errors."""
     from magma_amd import iteref
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
@@ -175,24 +176,26 @@ def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n
 
     out = {}
     for facto in ("bf16", "fp16"):
-        X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver="gmres", itermax=100)   # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver="gmres", itermax=100)
-        torch.cuda.synchronize()
-        tir = (time.perf_counter() - t0) / steps
-        be = float(backward_errors(torch, A, X, B).max())
-        fe = float(((X - X64).abs().amax(dim=(1, 2)) / X64.abs().amax(dim=(1, 2))).max())
-        itc = it.cpu()
-        out[facto] = {"solves_per_s": round(batch / tir, 1), "ms": round(tir * 1e3, 3),
-                      "gflops_fp64_credited": round(flop / tir / 1e9, 1),
-                      "iters_mean": round(float(itc.clamp(min=0).double().mean()), 2),
-                      "fallbacks": int((itc < 0).sum()), "backward_error_max": be,
-                      "forward_error_vs_fp64_max": fe, "speedup_vs_fp64": round(t64 / tir, 3)}
+        for solver in ("ir", "gmres"):
+            X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver=solver, itermax=100)   # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                X, it, info = iteref.dxgesv_iteref_batched(A, B, facto=facto, solver=solver, itermax=100)
+            torch.cuda.synchronize()
+            tir = (time.perf_counter() - t0) / steps
+            be = float(backward_errors(torch, A, X, B).max())
+            fe = float(((X - X64).abs().amax(dim=(1, 2)) / X64.abs().amax(dim=(1, 2))).max())
+            itc = it.cpu()
+            out[f"{facto}_{solver}"] = {
+                "solves_per_s": round(batch / tir, 1), "ms": round(tir * 1e3, 3),
+                "gflops_fp64_credited": round(flop / tir / 1e9, 1),
+                "iters_mean": round(float(itc.clamp(min=0).double().mean()), 2),
+                "iters_max": int(itc.max()), "fallbacks": int((itc < 0).sum()), "backward_error_max": be,
+                "forward_error_vs_fp64_max": fe, "speedup_vs_fp64": round(t64 / tir, 3)}
     res.update({"fp64_dgesv": {"solves_per_s": round(batch / t64, 1), "ms": round(t64 * 1e3, 3),
                                "gflops": round(flop / t64 / 1e9, 1), "backward_error_max": be64},
-                "lu_gmres_ir": out})
+                "lu_refinement": out})
     return res
 
 

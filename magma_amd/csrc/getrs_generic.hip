@@ -188,6 +188,171 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     }
 }
 
+// ---------------------------------------------------------------------------
+// NoTrans for n <= 64 R (n <= 256 in FP64, 512 in FP32), one wave per
+// (system, group of up to kWaveCols
+// right-hand sides): lane l keeps rows l + 64 s (s < R) of x in registers,
+// so the sweeps need neither LDS nor a workgroup barrier. Row k's solved
+// value reaches the other lanes through a cross-lane read, and the factor is
+// streamed down its columns (64 contiguous elements per slot, loads issued
+// ahead of the dependent updates). The interchanges are applied by tracing
+// every row back through them, as in the kernel above; the per-row operation
+// order is that of the column-oriented sweeps above.
+// ---------------------------------------------------------------------------
+constexpr int kWaveCols = 4;
+
+template <typename T, int R>
+__global__ void __launch_bounds__(kRsThreads)
+getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
+                  T** __restrict__ dB_array)
+{
+    const int gw = blockIdx.x * (kRsThreads / kWave) + threadIdx.x / kWave;
+    const int sys = gw / groups;
+    if (sys >= nsys) return;
+    const int b = sz.map ? sz.map[sys] : sys;
+    int n, nrhs, ldda, lddb;
+    sz.get(b, n, nrhs, ldda, lddb);
+    const int c0 = (gw % groups) * kWaveCols;
+    if (n <= 0 || c0 >= nrhs) return;
+    const int nr = min(kWaveCols, nrhs - c0);
+    const int l = threadIdx.x % kWave;
+    const T* A = dA_array[b];
+    const int* ipiv = ipiv_array[b];
+    T* B = dB_array[b] + (size_t)c0 * lddb;
+
+    // (P^T B)(i,:) = B(src(i),:)
+    int src[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) src[s] = l + 64 * s;
+    for (int k = n - 1; k >= 0; --k) {
+        int p = ipiv[k] - 1;
+        p = (p > k && p < n) ? p : k;
+#pragma unroll
+        for (int s = 0; s < R; ++s) src[s] = (src[s] == k) ? p : (src[s] == p ? k : src[s]);
+    }
+    T x[R][kWaveCols];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+        for (int c = 0; c < kWaveCols; ++c)
+            x[s][c] = (l + 64 * s < n && c < nr) ? B[src[s] + (size_t)c * lddb] : T(0);
+
+    // Columns are processed in chunks of CH; the factor entries of the next
+    // chunk are loaded while the current one is applied.
+    constexpr int CH = R <= 2 ? 8 : (R <= 4 ? 4 : 2);
+    // unit lower, forward: step k = 64 s + kk
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int kend = min(64, n - 1 - 64 * s);
+        if (kend <= 0) continue;
+        T cur[CH][R], nxt[CH][R];
+        auto load_l = [&](T (&buf)[CH][R], int kk0) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int k = 64 * s + kk0 + j;
+                const bool kin = kk0 + j < kend;
+                const T* Lk = A + (size_t)(kin ? k : 0) * ldda;
+#pragma unroll
+                for (int s2 = s; s2 < R; ++s2) {
+                    const int i = l + 64 * s2;
+                    buf[j][s2] = (kin && i > k && i < n) ? Lk[i] : T(0);
+                }
+            }
+        };
+        load_l(cur, 0);
+        for (int kk0 = 0; kk0 < kend; kk0 += CH) {
+            if (kk0 + CH < kend) load_l(nxt, kk0 + CH);
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int kk = kk0 + j;
+                if (kk < kend) {
+                    const int k = 64 * s + kk;
+#pragma unroll
+                    for (int c = 0; c < kWaveCols; ++c) {
+                        if (c < nr) {
+                            const T xk = readlane_t(x[s][c], kk);
+#pragma unroll
+                            for (int s2 = s; s2 < R; ++s2)
+                                if (l + 64 * s2 > k) x[s2][c] = fma(-cur[j][s2], xk, x[s2][c]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j)
+#pragma unroll
+                for (int s2 = s; s2 < R; ++s2) cur[j][s2] = nxt[j][s2];
+        }
+    }
+    // non-unit upper, backward: step k = 64 s + kk, kk descending
+#pragma unroll
+    for (int s = R - 1; s >= 0; --s) {
+        if (64 * s >= n) continue;
+        const int ktop = min(63, n - 1 - 64 * s);        // first kk processed
+        T cur[CH][R], nxt[CH][R], dcur[CH], dnxt[CH];
+        auto load_u = [&](T (&buf)[CH][R], T (&dg)[CH], int kk0) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int kk = kk0 - j;
+                const bool kin = kk >= 0;
+                const int k = 64 * s + (kin ? kk : 0);
+                const T* Uk = A + (size_t)k * ldda;
+                dg[j] = Uk[k];
+#pragma unroll
+                for (int s2 = 0; s2 <= s; ++s2) {
+                    const int i = l + 64 * s2;
+                    buf[j][s2] = (kin && i < k) ? Uk[i] : T(0);
+                }
+            }
+        };
+        load_u(cur, dcur, ktop);
+        for (int kk0 = ktop; kk0 >= 0; kk0 -= CH) {
+            if (kk0 - CH >= 0) load_u(nxt, dnxt, kk0 - CH);
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int kk = kk0 - j;
+                if (kk >= 0) {
+                    const int k = 64 * s + kk;
+#pragma unroll
+                    for (int c = 0; c < kWaveCols; ++c) {
+                        if (c < nr) {
+                            const T xk = readlane_t(x[s][c], kk) / dcur[j];
+                            if (l == kk) x[s][c] = xk;
+#pragma unroll
+                            for (int s2 = 0; s2 <= s; ++s2)
+                                if (l + 64 * s2 < k) x[s2][c] = fma(-cur[j][s2], xk, x[s2][c]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                dcur[j] = dnxt[j];
+#pragma unroll
+                for (int s2 = 0; s2 <= s; ++s2) cur[j][s2] = nxt[j][s2];
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+        for (int c = 0; c < kWaveCols; ++c)
+            if (l + 64 * s < n && c < nr) B[l + 64 * s + (size_t)c * lddb] = x[s][c];
+}
+
+template <typename T, int R>
+static magma_int_t getrs_wave_launch(const RsSizes& sz, int max_nrhs, T** dA_array, int** ipiv_array, T** dB_array,
+                                     int batch, hipStream_t stream)
+{
+    const int groups = ceildiv(max_nrhs, kWaveCols);
+    const long waves = (long)batch * groups;
+    const int wpb = kRsThreads / kWave;
+    getrs_wave_kernel<T, R><<<(int)ceildiv(waves, (long)wpb), kRsThreads, 0, stream>>>(sz, groups, batch, dA_array,
+                                                                                     ipiv_array, dB_array);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 size_t getrs_generic_bytes(int n, int elem) { return (size_t)n * kRsCols * elem + 2 * (size_t)n * sizeof(int); }
 size_t getrs_generic_bytes(int n) { return getrs_generic_bytes(n, sizeof(double)); }
 
@@ -195,6 +360,14 @@ template <typename T>
 magma_int_t getrs_generic_launch_t(int trans, const RsSizes& sz, int max_nrhs, T** dA_array, int** ipiv_array,
                                    T** dB_array, int batch, hipStream_t stream)
 {
+    if (batch <= 0 || max_nrhs <= 0) return MAGMA_SUCCESS;
+    if (trans == 0 && sz.max_n <= (sizeof(T) == 8 ? 256 : 512)) {
+        if (sz.max_n <= 64) return getrs_wave_launch<T, 1>(sz, max_nrhs, dA_array, ipiv_array, dB_array, batch, stream);
+        if (sz.max_n <= 128) return getrs_wave_launch<T, 2>(sz, max_nrhs, dA_array, ipiv_array, dB_array, batch, stream);
+        if (sz.max_n <= 256) return getrs_wave_launch<T, 4>(sz, max_nrhs, dA_array, ipiv_array, dB_array, batch, stream);
+        if constexpr (sizeof(T) == 4)
+            return getrs_wave_launch<T, 8>(sz, max_nrhs, dA_array, ipiv_array, dB_array, batch, stream);
+    }
     const size_t lds = getrs_generic_bytes(sz.max_n, sizeof(T));
     if (lds > max_dynamic_lds()) return MAGMA_ERR_NOT_SUPPORTED;
     MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T>,

@@ -31,6 +31,7 @@ namespace mgi {
 
 constexpr int kIrThreads = 256;
 constexpr int kGmresMaxRestart = 40;
+constexpr int kGmresCheck = 4;       // inner steps between host checks of the open cycles
 
 struct IrState {
     int* done;          // 1 converged, -1 diverged or not converged, 0 active
@@ -237,6 +238,7 @@ struct GmState {
     double* ro;      // current residual estimate
     int* inner;      // columns of the current cycle
     int* cycle_done; // 1 once the current cycle has stopped
+    int* cycle_open; // [0] systems whose current cycle is still open
     int* total;      // total inner iterations
     int m;           // restart length
 };
@@ -288,6 +290,7 @@ gm_start_kernel(int trans, int n, int col, double** __restrict__ dA_array, int l
         gs.inner[s] = 0;
         // a zero residual needs no cycle; a non-finite one ends in the fallback
         gs.cycle_done[s] = (beta_s == 0.0 || !isfinite(beta_s)) ? 1 : 0;
+        if (gs.cycle_done[s] == 0) atomicAdd(gs.cycle_open, 1);
     }
     __syncthreads();
     const double beta = beta_s;
@@ -391,7 +394,10 @@ gm_arnoldi_kernel(int trans, int n, int i, double** __restrict__ dA_array, int l
         gs.ro[s] = ro;
         gs.inner[s] = i + 1;
         gs.total[s] += 1;
-        if (ro <= gs.eps1[s] || i + 1 >= m || gs.total[s] >= maxiter) gs.cycle_done[s] = 1;
+        if (ro <= gs.eps1[s] || i + 1 >= m || gs.total[s] >= maxiter) {
+            gs.cycle_done[s] = 1;
+            atomicSub(gs.cycle_open, 1);
+        }
     }
 }
 
@@ -548,6 +554,7 @@ magma_int_t iteref_batched(int trans, int n, int nrhs, double** dA_array, int ld
         gs.inner = gi;
         gs.cycle_done = gi + batch;
         gs.total = gi + 2 * batch;
+        gs.cycle_open = ints + 7 * batch + 1;
         gs.m = m;
     }
     const double eps = DBL_EPSILON / 2;   // dlamch('Epsilon'): relative machine precision
@@ -606,11 +613,20 @@ magma_int_t iteref_batched(int trans, int n, int nrhs, double** dA_array, int ld
             gm_reset_kernel<<<g256, 256, 0, stream>>>(batch, nrhs, st, gs);
             int first = 1;
             for (int cycle = 0; cycle <= itermax; ++cycle) {
+                MGI_HIP_CHECK(hipMemsetAsync(gs.cycle_open, 0, sizeof(int), stream));
                 gm_start_kernel<<<batch, kIrThreads, xs_lds, stream>>>(trans, n, col, dA_array, ldda, dB_array, lddb,
                                                                        dX_array, lddx, st, gs, eps, first);
                 MGI_LAUNCH_CHECK();
                 first = 0;
                 for (int i = 0; i < m; ++i) {
+                    // every few inner steps: stop the cycle once every system's is closed
+                    if (i > 0 && i % kGmresCheck == 0) {
+                        int h_open = 0;
+                        MGI_HIP_CHECK(hipMemcpyAsync(&h_open, gs.cycle_open, sizeof(int), hipMemcpyDeviceToHost,
+                                                     stream));
+                        MGI_HIP_CHECK(hipStreamSynchronize(stream));
+                        if (h_open == 0) break;
+                    }
                     gm_v2s_kernel<<<dim3(std::max(1, ceildiv(n, 256)), batch), 256, 0, stream>>>(
                         n, gs.V, m, i, Rs, st.nrhs_eff, gs.cycle_done, solve_flag);
                     rc = getrs_generic_launch_t<float>(trans, rs1, 1, As_ptr, ipiv_ptr, Rs_ptr, batch, stream);

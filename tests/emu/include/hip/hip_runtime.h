@@ -86,6 +86,7 @@ inline void __syncthreads() { emu::block_sync(); }
 // a wave executes in lockstep: its scheduling barrier is a rendezvous here
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
 #define __builtin_amdgcn_fence(...) ((void)0)
+#define __builtin_amdgcn_sched_barrier(...) ((void)0)
 
 template <class T>
 inline T emu_shfl(T v, int src)

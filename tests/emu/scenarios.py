@@ -205,6 +205,9 @@ def main():
         elif w.startswith("getrf:"):
             _, m, n = w.split(":")
             scenario_getrf(q, int(m), int(n), batch=1)
+        elif w.startswith("sgesv:"):
+            _, n, r = w.split(":")
+            scenario_gesv(q, int(n), int(r), prec="s")
         elif w == "lu":
             lu_suite(q)
         elif w == "overflow":
@@ -217,8 +220,8 @@ def main():
             _, n, facto, solver = w.split(":")
             scenario_dhgesv(int(n), facto, solver)
         elif w.startswith("acc:"):
-            _, facto, solver, cond = w.split(":")
-            scenario_accuracy(q, facto, solver, float(cond))
+            f = w.split(":")
+            scenario_accuracy(q, f[1], f[2], float(f[3]), **({"n": int(f[4]), "batch": 2} if len(f) > 4 else {}))
         else:
             raise SystemExit("unknown scenario " + w)
 
@@ -311,7 +314,7 @@ def lu_suite(q):
     for n, r in ((8, 3), (33, 4), (128, 9), (200, 2)):
         for t in (0, 1):
             scenario_getrs(q, n, r, t)
-    for n, r in ((16, 1), (64, 3), (128, 4), (200, 2), (300, 5)):
+    for n, r in ((16, 1), (64, 3), (128, 4), (129, 2), (200, 2), (256, 3), (300, 5)):
         scenario_gesv(q, n, r, prec="s")
     for m, n in ((60, 60), (200, 200), (300, 300)):
         scenario_getrf(q, m, n, prec="s")

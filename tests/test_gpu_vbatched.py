@@ -198,7 +198,7 @@ def test_vbatched_plan_graph_replay(mb, oracle, variant):
     plan.close()
 
 
-@pytest.mark.parametrize("n,nrhs", [(16, 1), (64, 3), (128, 4), (200, 2), (300, 5)])
+@pytest.mark.parametrize("n,nrhs", [(16, 1), (64, 3), (128, 4), (129, 2), (200, 2), (241, 3), (256, 4), (300, 5)])
 def test_sgesv_batched(mb, oracle, n, nrhs):
     rng = np.random.default_rng(10 + n)
     batch = 16
