@@ -274,8 +274,10 @@ magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
         magmaDouble_ptr dworkd, magmaFloat_ptr dworks, magma_int_t *iter, magma_int_t *info);
 
 /* No MAGMA counterpart: batched mixed-precision solve of A_i X_i = B_i
- * (n <= 256, any nrhs), one fused launch per batch. A_i and B_i are read only;
- * X_i is written. facto_type selects the low-precision LU (SHT, SBT, SB3T, SS),
+ * (n <= 2048, any nrhs; the low-precision LU, the refinement steps and the
+ * FP64 fallback are each one batched launch sequence on the queue's stream,
+ * and the call returns after the refinement has finished). A_i and B_i are
+ * read only; X_i is written. facto_type selects the low-precision LU (SHT, SBT, SB3T, SS),
  * solver_type IRSTRS or GMSTRS; itermax bounds refinement steps.
  * iter_array[i] follows the iter convention of magma_dhgesv_iteref_gpu;
  * a system that does not converge is re-solved in FP64 inside the launch.
