@@ -24,9 +24,18 @@
 #include "lu_device.h"
 #include "getrf_kernels.h"
 
+#include <type_traits>
+#include <utility>
+
 namespace mgi {
 
 constexpr int kPanelThreads = 256;
+
+template <int... Js, typename F>
+__device__ __forceinline__ void static_for(std::integer_sequence<int, Js...>, F&& f)
+{
+    (f(std::integral_constant<int, Js>{}), ...);
+}
 
 template <typename T> struct Limits;
 template <> struct Limits<double> { static constexpr double sfmin = kSfminD; };
@@ -70,9 +79,10 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
     }
 
     int info = 0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        // constant trip count (cross-lane operations inside)
+    // every column step is instantiated with its index as a constant (a
+    // run-time column index would move the register panel to scratch memory)
+    static_for(std::make_integer_sequence<int, NB>{}, [&](auto jc) {
+        constexpr int j = decltype(jc)::value;
         if (j < jb) {
             T best = T(-1);
             int bi = 0x7fffffff;
@@ -149,7 +159,7 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
             }
             __syncthreads();
         }
-    }
+    });
 
 #pragma unroll
     for (int i = 0; i < R; ++i) {
@@ -475,13 +485,26 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
                                  hipStream_t stream)
 {
     for (int k = 0; k < max_minmn; k += NB) {
+        // only panels of R * NB <= 64 register values per thread exist
+        // (blocked_nb_valid)
+        auto panel = [&](auto rc) -> magma_int_t {
+            constexpr int RR = decltype(rc)::value;
+            if constexpr (RR * NB <= 64) {
+                panel_kernel<T, RR, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array);
+                return MAGMA_SUCCESS;
+            } else {
+                return MAGMA_ERR_NOT_SUPPORTED;
+            }
+        };
+        magma_int_t prc = MAGMA_ERR_NOT_SUPPORTED;
         switch (R) {
-            case 1: panel_kernel<T, 1, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 2: panel_kernel<T, 2, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 4: panel_kernel<T, 4, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            case 8: panel_kernel<T, 8, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array); break;
-            default: return MAGMA_ERR_NOT_SUPPORTED;
+            case 1: prc = panel(std::integral_constant<int, 1>{}); break;
+            case 2: prc = panel(std::integral_constant<int, 2>{}); break;
+            case 4: prc = panel(std::integral_constant<int, 4>{}); break;
+            case 8: prc = panel(std::integral_constant<int, 8>{}); break;
+            default: break;
         }
+        if (prc != MAGMA_SUCCESS) return prc;
         MGI_LAUNCH_CHECK();
         const int other = max_n;   // upper bound of the columns outside the panel
         laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
