@@ -175,12 +175,13 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
 
 // ---------------------------------------------------------------------------
 // 2. Row interchanges of panel k..k+jb-1 on every column outside the panel.
-//    The jb interchanges touch at most 2*jb rows: the panel rows k..k+jb-1
-//    and the pivot rows below them. Lane i of each wave takes one touched
-//    destination row and traces it back through the interchanges (last to
-//    first) to the row whose content ends there; the wave then moves one
-//    column at a time, every lane loading its source before any lane stores
-//    (the composition equals applying the interchanges in order, as dlaswp).
+//    The jb <= 64 interchanges touch at most 2*jb rows: the panel rows
+//    k..k+jb-1 and the pivot rows below them. Each lane takes up to two
+//    touched destination rows and traces each back through the interchanges
+//    (last to first) to the row whose content ends there; the wave then moves
+//    one column at a time, every lane loading its sources before any lane
+//    stores (the composition equals applying the interchanges in order, as
+//    dlaswp).
 //    Interchanges outside [k+j, m) cannot come from a valid panel and are
 //    treated as none.
 // ---------------------------------------------------------------------------
@@ -190,7 +191,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __restrict__ ipiv_array)
 {
-    __shared__ int s_piv[32];
+    __shared__ int s_piv[64];
     const int b = sz.mat(blockIdx.y);
     int m, n, ld;
     sz.get(b, m, n, ld);
@@ -206,28 +207,32 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
         s_piv[t] = (p > k + t && p < m) ? p : k + t;
     }
     __syncthreads();
+    // lane l covers panel row k + l and the pivot row of step l (when that
+    // lies below the panel; a pivot row inside the panel has its own lane)
     const int l = t % kWave, w = t / kWave;
-    int dst = -1;
+    int dst0 = -1, dst1 = -1;
     if (l < jb) {
-        dst = k + l;
-    } else if (l < 2 * jb) {
-        dst = s_piv[l - jb];
-        if (dst < k + jb) dst = -1;                 // a panel row: already a lane of its own
+        dst0 = k + l;
+        dst1 = s_piv[l];
+        if (dst1 < k + jb) dst1 = -1;
     }
-    int src = dst;
+    int src0 = dst0, src1 = dst1;
     for (int j = jb - 1; j >= 0; --j) {
         const int p = s_piv[j];
-        src = (src == k + j) ? p : (src == p ? k + j : src);
+        src0 = (src0 == k + j) ? p : (src0 == p ? k + j : src0);
+        src1 = (src1 == k + j) ? p : (src1 == p ? k + j : src1);
     }
-    const bool move = dst >= 0 && src != dst;
+    const bool mv0 = dst0 >= 0 && src0 != dst0, mv1 = dst1 >= 0 && src1 != dst1;
     const int cend = min(other, cbeg + kLaswpCols);
     for (int c = cbeg + w; c < cend; c += 256 / kWave) {
         const int col = c < k ? c : c + jb;
         T* A = dA_array[b] + (size_t)col * ld;
-        T v = T(0);
-        if (move) v = A[src];
+        T v0 = T(0), v1 = T(0);
+        if (mv0) v0 = A[src0];
+        if (mv1) v1 = A[src1];
         __builtin_amdgcn_wave_barrier();            // all sources read before any store
-        if (move) A[dst] = v;
+        if (mv0) A[dst0] = v0;
+        if (mv1) A[dst1] = v1;
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -302,7 +307,7 @@ __global__ void __launch_bounds__(256)
 gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
 {
     constexpr bool kWide = (MODE == kUpdBF16 || MODE == kUpdFP16 || MODE == kUpdBF16x3);
-    constexpr int KP = kWide ? 32 : NB;              // K staged (zero padded)
+    constexpr int KP = kWide ? (NB + 31) / 32 * 32 : NB;   // K staged (zero padded to whole MFMA steps)
     __shared__ T sA[KP][64 + 1];                     // A21 tile, [kk][row]
     __shared__ T sB[KP][64 + 1];                     // A12 tile, [kk][col]
     const int b = sz.mat(blockIdx.z);
@@ -389,52 +394,55 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
                 }
             }
         } else {
-            // one K=32 step: lane holds k = 8*(l/16) + v, v < 8
-            float av[2][8], bv[2][8];
+            // K = 32 per MFMA step: lane holds k = kb + 8*(l/16) + v, v < 8
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti) av[ti][v] = sA[8 * lq + v][wr + 16 * ti + lr];
-#pragma unroll
-                for (int tj = 0; tj < 2; ++tj) bv[tj][v] = sB[8 * lq + v][wc + 16 * tj + lr];
-            }
-            if constexpr (MODE == kUpdFP16) {
-                f16x8_t ah[2], bh[2];
+            for (int kb = 0; kb < KP; kb += 32) {
+                float av[2][8], bv[2][8];
 #pragma unroll
                 for (int v = 0; v < 8; ++v) {
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        ah[q][v] = (_Float16)av[q][v];
-                        bh[q][v] = (_Float16)bv[q][v];
-                    }
+                    for (int ti = 0; ti < 2; ++ti) av[ti][v] = sA[kb + 8 * lq + v][wr + 16 * ti + lr];
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) bv[tj][v] = sB[kb + 8 * lq + v][wc + 16 * tj + lr];
                 }
+                if constexpr (MODE == kUpdFP16) {
+                    f16x8_t ah[2], bh[2];
 #pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
+                    for (int v = 0; v < 8; ++v) {
 #pragma unroll
-                    for (int tj = 0; tj < 2; ++tj)
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
-            } else {
-                bf16x8_t ah[2], bh[2], al[2], bl[2];
-#pragma unroll
-                for (int v = 0; v < 8; ++v) {
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        ah[q][v] = (__bf16)av[q][v];
-                        bh[q][v] = (__bf16)bv[q][v];
-                        al[q][v] = (__bf16)(av[q][v] - (float)ah[q][v]);
-                        bl[q][v] = (__bf16)(bv[q][v] - (float)bh[q][v]);
-                    }
-                }
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < 2; ++tj) {
-                        if constexpr (MODE == kUpdBF16x3) {
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ti], bh[tj], acc[ti][tj], 0, 0, 0);
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bl[tj], acc[ti][tj], 0, 0, 0);
+                        for (int q = 0; q < 2; ++q) {
+                            ah[q][v] = (_Float16)av[q][v];
+                            bh[q][v] = (_Float16)bv[q][v];
                         }
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
                     }
+#pragma unroll
+                    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < 2; ++tj)
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                } else {
+                    bf16x8_t ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            ah[q][v] = (__bf16)av[q][v];
+                            bh[q][v] = (__bf16)bv[q][v];
+                            al[q][v] = (__bf16)(av[q][v] - (float)ah[q][v]);
+                            bl[q][v] = (__bf16)(bv[q][v] - (float)bh[q][v]);
+                        }
+                    }
+#pragma unroll
+                    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < 2; ++tj) {
+                            if constexpr (MODE == kUpdBF16x3) {
+                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bl[tj], acc[ti][tj], 0, 0, 0);
+                            }
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                        }
+                }
             }
         }
 #pragma unroll
@@ -533,7 +541,7 @@ bool blocked_nb_valid(int max_m, int nb)
 {
     BlockedPlan p;
     if (!plan_for(max_m, p)) return false;
-    return (nb == 8 || nb == 16 || nb == 32) && p.rows_per_thread * nb <= 64;
+    return (nb == 8 || nb == 16 || nb == 32 || nb == 64) && p.rows_per_thread * nb <= 64;
 }
 
 template <typename T, int MODE>
@@ -550,6 +558,7 @@ static magma_int_t blocked_launch(const SizeArgs& sz, int max_m, int max_n, T** 
     MGI_LAUNCH_CHECK();
     const int max_minmn = std::min(max_m, max_n);
     switch (plan.nb) {
+        case 64: return blocked_steps<T, 64, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 32: return blocked_steps<T, 32, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 16: return blocked_steps<T, 16, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 8:  return blocked_steps<T, 8, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);

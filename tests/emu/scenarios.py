@@ -219,6 +219,8 @@ def main():
         elif w.startswith("dhgesv:"):
             _, n, facto, solver = w.split(":")
             scenario_dhgesv(int(n), facto, solver)
+        elif w.startswith("vbplan:"):
+            scenario_vbplan(q, int(w.split(":")[1]))
         elif w.startswith("acc:"):
             f = w.split(":")
             scenario_accuracy(q, f[1], f[2], float(f[3]), **({"n": int(f[4]), "batch": 2} if len(f) > 4 else {}))
@@ -302,6 +304,43 @@ def scenario_getrs(q, n, nrhs, trans, batch=2):
              for s in range(batch))
     print(f"dgetrs n={n} nrhs={nrhs} trans={trans} rc {rc} be {be:.2e}", flush=True)
     check(rc == 0 and be < 1e-14, f"dgetrs {n} {nrhs} {trans}")
+
+
+def scenario_vbplan(q, variant, ns=(129, 140, 200, 256, 177, 64, 30, 300), rs=(1, 3, 16, 2, 5, 4, 1, 2)):
+    """Variable-size plan executed with one kernel choice for every bucket
+    (bucket-invalid choices fall back to the built-in rule)."""
+    rng = np.random.default_rng(variant * 13 + len(ns))
+    batch = len(ns)
+    mats, rhs, bA, bB = [], [], [], []
+    for n, r in zip(ns, rs):
+        M = rng.uniform(-1, 1, (n, n)); R = rng.uniform(-1, 1, (n, r))
+        mats.append(M); rhs.append(R)
+        bA.append(Buf(np.ascontiguousarray(M.T))); bB.append(Buf(np.ascontiguousarray(R.T)))
+    bP = [Buf(np.zeros(n, np.int32)) for n in ns]
+    pA = Buf(np.array([x.ptr for x in bA], dtype=np.uint64))
+    pB = Buf(np.array([x.ptr for x in bB], dtype=np.uint64))
+    pP = Buf(np.array([x.ptr for x in bP], dtype=np.uint64))
+    dn = Buf(np.array(ns, np.int32)); dr = Buf(np.array(rs, np.int32)); info = Buf(np.zeros(batch, np.int32))
+    hn = np.array(ns, np.int32); hr = np.array(rs, np.int32); hv = np.full(22, variant, np.int32)
+    plan = vp()
+    LIB.magma_dgesv_vbatched_plan_create.argtypes = [vp, vp, i32, vp, vp, ctypes.POINTER(vp)]
+    rc = LIB.magma_dgesv_vbatched_plan_create(hn.ctypes.data, hr.ctypes.data, batch, hv.ctypes.data, q,
+                                              ctypes.byref(plan))
+    check(rc == 0, f"vbplan create {variant}")
+    LIB.magma_dgesv_vbatched_plan_exec.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    rc = LIB.magma_dgesv_vbatched_plan_exec(plan, dn.ptr, dr.ptr, pA.ptr, dn.ptr, pP.ptr, pB.ptr, dn.ptr,
+                                            info.ptr, q)
+    LIB.magma_queue_sync_internal.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, i32]
+    LIB.magma_queue_sync_internal(q, b"", b"", 0)
+    LIB.magma_vbatched_plan_destroy.argtypes = [vp]
+    LIB.magma_vbatched_plan_destroy(plan)
+    worst = 0.0
+    for i, (n, r) in enumerate(zip(ns, rs)):
+        X = bB[i].get().T
+        be = np.abs(rhs[i] - mats[i] @ X).sum(1).max() / (np.abs(mats[i]).sum(1).max() * np.abs(X).sum(1).max())
+        worst = max(worst, be)
+    print(f"vbplan variant {variant} rc {rc} info {info.get()} be {worst:.2e}", flush=True)
+    check(rc == 0 and (info.get() == 0).all() and worst < 1e-14, f"vbplan {variant}")
 
 
 def lu_suite(q):
