@@ -53,10 +53,14 @@ __global__ void __launch_bounds__(kPanelThreads)
 panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
              int* __restrict__ info_array)
 {
-    __shared__ T s_val[kPanelThreads / kWave];
-    __shared__ int s_idx[kPanelThreads / kWave];
-    __shared__ T s_rowj[NB];
-    __shared__ T s_rowp[NB];
+    constexpr int kW = kPanelThreads / kWave;
+    // per column step (double-buffered by step parity, so one barrier per
+    // step suffices): each wave's best candidate with its whole panel row,
+    // and row j as it was before the interchange
+    __shared__ T s_val[2][kW];
+    __shared__ int s_idx[2][kW];
+    __shared__ T s_cand[2][kW][NB];
+    __shared__ T s_rowj[2][NB];
 
     const int b = sz.mat(blockIdx.x);
     int m, n, ld;
@@ -66,6 +70,7 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
     const int jb = min(NB, minmn - k);
     const int mloc = m - k;
     const int t = threadIdx.x;
+    const int w = t / kWave;
     T* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
     int* ipiv = ipiv_array[b];
 
@@ -83,9 +88,10 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
     // run-time column index would move the register panel to scratch memory)
     static_for(std::make_integer_sequence<int, NB>{}, [&](auto jc) {
         constexpr int j = decltype(jc)::value;
+        constexpr int par = j & 1;
         if (j < jb) {
             T best = T(-1);
-            int bi = 0x7fffffff;
+            int bi = 0x7fffffff, bs = 0;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 const int r = t + kPanelThreads * i;
@@ -94,36 +100,43 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
                     // first candidate, which keeps the pivot (counted as +inf)
                     T v = fabs(a[i][j]);
                     if (r == j && v != v) v = T(INFINITY);
-                    if (v > best) { best = v; bi = r; }
+                    if (v > best) { best = v; bi = r; bs = i; }
                 }
             }
-            wave_argmax_t(best, bi);
-            const int w = t / kWave;
-            if ((t & (kWave - 1)) == 0) { s_val[w] = best; s_idx[w] = bi; }
-            __syncthreads();
-            best = s_val[0];
-            bi = s_idx[0];
+            T wb = best;
+            int wi = bi;
+            wave_argmax_t(wb, wi);
+            // the holder of the wave's best publishes it with its row; the
+            // holder of row j (thread j, slot 0) publishes row j
+            if (bi == wi && best == wb && wi != 0x7fffffff) {
+                s_val[par][w] = wb;
+                s_idx[par][w] = wi;
 #pragma unroll
-            for (int q = 1; q < kPanelThreads / kWave; ++q) {
-                T ov = s_val[q];
-                int oi = s_idx[q];
-                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+                for (int i = 0; i < R; ++i)
+                    if (bs == i) {
+#pragma unroll
+                        for (int c = 0; c < NB; ++c) s_cand[par][w][c] = a[i][c];
+                    }
+            } else if ((t & (kWave - 1)) == 0 && wi == 0x7fffffff) {
+                s_val[par][w] = T(-1);
+                s_idx[par][w] = 0x7fffffff;
             }
-            const int p = bi;
-            // Row j (thread j, slot 0) and row p (thread p % 256, slot p / 256)
-            // publish their panel rows.
             if (t == j) {
 #pragma unroll
-                for (int c = 0; c < NB; ++c) s_rowj[c] = a[0][c];
+                for (int c = 0; c < NB; ++c) s_rowj[par][c] = a[0][c];
             }
-#pragma unroll
-            for (int i = 0; i < R; ++i)
-                if (t + kPanelThreads * i == p) {
-#pragma unroll
-                    for (int c = 0; c < NB; ++c) s_rowp[c] = a[i][c];
-                }
             __syncthreads();
-            const T pv = s_rowp[j];
+            T pbest = s_val[par][0];
+            int p = s_idx[par][0], pw = 0;
+#pragma unroll
+            for (int q = 1; q < kW; ++q) {
+                const T ov = s_val[par][q];
+                const int oi = s_idx[par][q];
+                if (ov > pbest || (ov == pbest && oi < p)) { pbest = ov; p = oi; pw = q; }
+            }
+            const T* prow = s_cand[par][pw];
+            const T* jrow = s_rowj[par];
+            const T pv = prow[j];
             if (t == 0) ipiv[k + j] = k + p + 1;
             T rinv = T(1);
             bool recip = true;
@@ -131,13 +144,13 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
                 if (p != j) {
                     if (t == j) {
 #pragma unroll
-                        for (int c = 0; c < NB; ++c) a[0][c] = s_rowp[c];
+                        for (int c = 0; c < NB; ++c) a[0][c] = prow[c];
                     }
 #pragma unroll
                     for (int i = 0; i < R; ++i)
                         if (t + kPanelThreads * i == p) {
 #pragma unroll
-                            for (int c = 0; c < NB; ++c) a[i][c] = s_rowj[c];
+                            for (int c = 0; c < NB; ++c) a[i][c] = jrow[c];
                         }
                 }
                 recip = fabs(pv) >= Limits<T>::sfmin;
@@ -154,10 +167,9 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
                     a[i][j] = l;
 #pragma unroll
                     for (int c = 0; c < NB; ++c)
-                        if (c > j && c < jb) a[i][c] = fma(-l, s_rowp[c], a[i][c]);
+                        if (c > j && c < jb) a[i][c] = fma(-l, prow[c], a[i][c]);
                 }
             }
-            __syncthreads();
         }
     });
 
@@ -541,7 +553,7 @@ bool blocked_nb_valid(int max_m, int nb)
 {
     BlockedPlan p;
     if (!plan_for(max_m, p)) return false;
-    return (nb == 8 || nb == 16 || nb == 32 || nb == 64) && p.rows_per_thread * nb <= 64;
+    return (nb == 8 || nb == 16 || nb == 32) && p.rows_per_thread * nb <= 64;
 }
 
 template <typename T, int MODE>
@@ -558,7 +570,6 @@ static magma_int_t blocked_launch(const SizeArgs& sz, int max_m, int max_n, T** 
     MGI_LAUNCH_CHECK();
     const int max_minmn = std::min(max_m, max_n);
     switch (plan.nb) {
-        case 64: return blocked_steps<T, 64, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 32: return blocked_steps<T, 32, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 16: return blocked_steps<T, 16, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 8:  return blocked_steps<T, 8, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
