@@ -204,7 +204,8 @@ magma_int_t magma_dgesv_vbatched(magma_int_t *n, magma_int_t *nrhs,
  * so executing it allocates nothing and never synchronises; it can be
  * recorded once into a hipGraph and replayed. h_variants (nullable) holds a
  * kernel choice per bucket id (0 auto, 1 register-tile, 2/3/4 blocked with
- * panel width 32/16/8), e.g. from magma_amd.autotune. h_nrhs == NULL makes
+ * panel width 32/16/8, 5 blocked with two 32-wide panels per 64-wide block,
+ * orders <= 512), e.g. from magma_amd.autotune. h_nrhs == NULL makes
  * a factor-only plan. */
 struct magma_vbatched_plan;
 typedef struct magma_vbatched_plan* magma_vbatched_plan_t;

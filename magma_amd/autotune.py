@@ -1,6 +1,7 @@
 """Autotuner for variable-size batches: picks, per n-bucket, the kernel
 variant of the vbatched plan (register-tiled LU for n <= 128, or the blocked
-LU with panel width 32, 16 or 8) by timing each valid variant on the bucket's
+LU with panel width 32, 16 or 8, or two 32-wide panels per 64-wide block) by
+timing each valid variant on the bucket's
 own systems, replayed as a hipGraph. Results are cached in a JSON file keyed
 by device name and bucket id; `load_variants` reads the cache and tunes only
 buckets that are missing from it.
@@ -19,7 +20,7 @@ import torch
 
 from .batched import IpivV, VBatch, VbatchedPlan, default_queue
 
-VARIANT_NAMES = {1: "tile", 2: "blocked nb=32", 3: "blocked nb=16", 4: "blocked nb=8"}
+VARIANT_NAMES = {1: "tile", 2: "blocked nb=32", 3: "blocked nb=16", 4: "blocked nb=8", 5: "blocked 2x32"}
 CACHE = Path(os.environ.get("MAGMA_AMD_TUNING", Path(__file__).resolve().parent / "tuning" / "vbatched.json"))
 
 
@@ -85,7 +86,7 @@ def tune_buckets(n, nrhs, buckets=None, max_systems: int = 4096, reps: int = 3, 
         sn, sr = n[idx], nrhs[idx]
         A, B, ipiv, info = _make_inputs(sn, sr, seed + b, dev)
         timings = {}
-        for v in (1, 2, 3, 4):
+        for v in (1, 2, 3, 4, 5):
             plan = VbatchedPlan(sn, sr, variants={b: v})
             got = [x for x in plan.buckets() if x["id"] == b]
             if not got or got[0]["variant"] != v:

@@ -198,10 +198,14 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
 //    treated as none.
 // ---------------------------------------------------------------------------
 constexpr int kLaswpCols = 32;     // columns per workgroup
+// Column ranges of the two-level blocked LU (64-wide blocks factored as two
+// 32-wide panels): see blocked_steps2.
+constexpr int kInPanelRight = 1;
+constexpr int kInPanelLeft = 2;
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __restrict__ ipiv_array)
+laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __restrict__ ipiv_array, int mode)
 {
     __shared__ int s_piv[64];
     const int b = sz.mat(blockIdx.y);
@@ -210,9 +214,21 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(nb, minmn - k);
-    const int other = n - jb;                       // columns outside the panel
+    // mode 0: every column outside the panel; kInPanelRight: the columns of
+    // the second half of a 64-wide block (panel = its first half); kInPanelLeft:
+    // the 32 columns left of the panel (panel = the block's second half)
+    int span = n - jb, lo = 0;
+    if (mode == kInPanelRight) {
+        const int W = min(2 * nb, minmn - k);
+        if (W <= nb) return;
+        lo = k + nb;
+        span = W - nb;
+    } else if (mode == kInPanelLeft) {
+        lo = k - nb;
+        span = nb;
+    }
     const int cbeg = blockIdx.x * kLaswpCols;
-    if (cbeg >= other) return;
+    if (cbeg >= span) return;
     const int t = threadIdx.x;
     if (t < jb) {
         const int p = ipiv_array[b][k + t] - 1;
@@ -235,9 +251,9 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
         src1 = (src1 == k + j) ? p : (src1 == p ? k + j : src1);
     }
     const bool mv0 = dst0 >= 0 && src0 != dst0, mv1 = dst1 >= 0 && src1 != dst1;
-    const int cend = min(other, cbeg + kLaswpCols);
+    const int cend = min(span, cbeg + kLaswpCols);
     for (int c = cbeg + w; c < cend; c += 256 / kWave) {
-        const int col = c < k ? c : c + jb;
+        const int col = mode == 0 ? (c < k ? c : c + jb) : lo + c;
         T* A = dA_array[b] + (size_t)col * ld;
         T v0 = T(0), v1 = T(0);
         if (mv0) v0 = A[src0];
@@ -259,7 +275,7 @@ constexpr int kTrsmCols = 64;      // columns per workgroup
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256)
-trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array)
+trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
 {
     constexpr int CPW = kWave / NB;                 // columns per wave pass
     __shared__ T sL[NB * NB];
@@ -269,7 +285,12 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array)
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(NB, minmn - k);
-    const int ncols = n - k - jb;
+    int ncols = n - k - jb;
+    if (mode == kInPanelRight) {                    // only the block's second half
+        const int W = min(2 * NB, minmn - k);
+        if (W <= NB) return;
+        ncols = W - NB;
+    }
     if (ncols <= 0) return;
     const int c0 = blockIdx.x * kTrsmCols;
     if (c0 >= ncols) return;
@@ -316,7 +337,7 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 
 template <typename T, int NB, int MODE>
 __global__ void __launch_bounds__(256)
-gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
+gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mode)
 {
     constexpr bool kWide = (MODE == kUpdBF16 || MODE == kUpdFP16 || MODE == kUpdBF16x3);
     constexpr int KP = kWide ? (NB + 31) / 32 * 32 : NB;   // K staged (zero padded to whole MFMA steps)
@@ -328,7 +349,13 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(nb, minmn - k);
-    const int mr = m - k - jb, nc = n - k - jb;
+    const int mr = m - k - jb;
+    int nc = n - k - jb;
+    if (mode == kInPanelRight) {                    // only the block's second half
+        const int W = min(2 * nb, minmn - k);
+        if (W <= nb) return;
+        nc = W - nb;
+    }
     const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
     if (r0 >= mr || c0 >= nc) return;
     T* A = dA_array[b];
@@ -499,45 +526,105 @@ static bool plan_for(int max_m, BlockedPlan& p)
     return true;
 }
 
+template <typename T, int NB>
+static magma_int_t panel_launch(int R, const SizeArgs& sz, int k, T** dA_array, int** ipiv_array, int* info_array,
+                                int batch, hipStream_t stream)
+{
+    // only panels of R * NB <= 64 register values per thread exist
+    // (blocked_nb_valid)
+    auto panel = [&](auto rc) -> magma_int_t {
+        constexpr int RR = decltype(rc)::value;
+        if constexpr (RR * NB <= 64) {
+            panel_kernel<T, RR, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array);
+            return MAGMA_SUCCESS;
+        } else {
+            return MAGMA_ERR_NOT_SUPPORTED;
+        }
+    };
+    magma_int_t prc = MAGMA_ERR_NOT_SUPPORTED;
+    switch (R) {
+        case 1: prc = panel(std::integral_constant<int, 1>{}); break;
+        case 2: prc = panel(std::integral_constant<int, 2>{}); break;
+        case 4: prc = panel(std::integral_constant<int, 4>{}); break;
+        case 8: prc = panel(std::integral_constant<int, 8>{}); break;
+        default: break;
+    }
+    if (prc != MAGMA_SUCCESS) return prc;
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 template <typename T, int NB, int MODE>
 static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n, int max_minmn,
                                  T** dA_array, int** ipiv_array, int* info_array, int batch,
                                  hipStream_t stream)
 {
     for (int k = 0; k < max_minmn; k += NB) {
-        // only panels of R * NB <= 64 register values per thread exist
-        // (blocked_nb_valid)
-        auto panel = [&](auto rc) -> magma_int_t {
-            constexpr int RR = decltype(rc)::value;
-            if constexpr (RR * NB <= 64) {
-                panel_kernel<T, RR, NB><<<batch, kPanelThreads, 0, stream>>>(sz, k, dA_array, ipiv_array, info_array);
-                return MAGMA_SUCCESS;
-            } else {
-                return MAGMA_ERR_NOT_SUPPORTED;
-            }
-        };
-        magma_int_t prc = MAGMA_ERR_NOT_SUPPORTED;
-        switch (R) {
-            case 1: prc = panel(std::integral_constant<int, 1>{}); break;
-            case 2: prc = panel(std::integral_constant<int, 2>{}); break;
-            case 4: prc = panel(std::integral_constant<int, 4>{}); break;
-            case 8: prc = panel(std::integral_constant<int, 8>{}); break;
-            default: break;
-        }
+        magma_int_t prc = panel_launch<T, NB>(R, sz, k, dA_array, ipiv_array, info_array, batch, stream);
         if (prc != MAGMA_SUCCESS) return prc;
-        MGI_LAUNCH_CHECK();
         const int other = max_n;   // upper bound of the columns outside the panel
         laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
-                                                                                   ipiv_array);
+                                                                                   ipiv_array, 0);
         MGI_LAUNCH_CHECK();
         const int rest_n = max_n - k - 1;
         const int rest_m = max_m - k - 1;
         if (rest_n > 0) {
-            trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array);
+            trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array, 0);
             MGI_LAUNCH_CHECK();
             if (rest_m > 0) {
                 dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
-                gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array);
+                gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array, 0);
+                MGI_LAUNCH_CHECK();
+            }
+        }
+    }
+    return MAGMA_SUCCESS;
+}
+
+// Two-level blocking: each 64-wide block is factored as two 32-wide panels
+// (the second half updated from the first inside the block), then the
+// interchanges, L11^{-1} and the trailing update are applied once for the
+// whole block (K = 64). Same arithmetic as the 32-wide blocked loop, with
+// half as many passes over the trailing matrix and over the row swaps.
+template <typename T, int MODE>
+static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_n, int max_minmn,
+                                  T** dA_array, int** ipiv_array, int* info_array, int batch,
+                                  hipStream_t stream)
+{
+    constexpr int H = 32;
+    for (int k = 0; k < max_minmn; k += 2 * H) {
+        // first half: panel, then its interchanges, U12 and update on the
+        // block's second half only
+        magma_int_t prc = panel_launch<T, H>(R, sz, k, dA_array, ipiv_array, info_array, batch, stream);
+        if (prc != MAGMA_SUCCESS) return prc;
+        laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k, H, dA_array, ipiv_array, kInPanelRight);
+        MGI_LAUNCH_CHECK();
+        trsm_kernel<T, H><<<dim3(1, batch), 256, 0, stream>>>(sz, k, dA_array, kInPanelRight);
+        MGI_LAUNCH_CHECK();
+        if (max_m - k - H > 0) {
+            gemm_update_kernel<T, H, MODE><<<dim3(ceildiv(max_m - k - H, 64), 1, batch), 256, 0, stream>>>(
+                sz, k, H, dA_array, kInPanelRight);
+            MGI_LAUNCH_CHECK();
+        }
+        // second half: panel, then its interchanges on the first half
+        if (k + H < max_minmn) {
+            prc = panel_launch<T, H>(R, sz, k + H, dA_array, ipiv_array, info_array, batch, stream);
+            if (prc != MAGMA_SUCCESS) return prc;
+            laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k + H, H, dA_array, ipiv_array, kInPanelLeft);
+            MGI_LAUNCH_CHECK();
+        }
+        // the whole block's interchanges outside it, U12 and the trailing update
+        laswp_kernel<T><<<dim3(ceildiv(max_n, kLaswpCols), batch), 256, 0, stream>>>(sz, k, 2 * H, dA_array,
+                                                                                   ipiv_array, 0);
+        MGI_LAUNCH_CHECK();
+        const int rest_n = max_n - k - 1;
+        const int rest_m = max_m - k - 1;
+        if (rest_n > 0) {
+            trsm_kernel<T, 2 * H><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array, 0);
+            MGI_LAUNCH_CHECK();
+            if (rest_m > 0) {
+                dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
+                gemm_update_kernel<T, 2 * H, MODE><<<grid, 256, 0, stream>>>(sz, k, 2 * H, dA_array, 0);
                 MGI_LAUNCH_CHECK();
             }
         }
@@ -548,11 +635,13 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
 bool getrf_blocked_supported(int max_m) { BlockedPlan p; return plan_for(max_m, p); }
 
 // Panel widths a given row count supports: the panel keeps R x NB values per
-// thread in registers, R = ceil(max_m / 256), and R * NB <= 64.
+// thread in registers, R = ceil(max_m / 256), and R * NB <= 64. Width 64
+// means the two-level form (two 32-wide panels per 64-wide block), R <= 2.
 bool blocked_nb_valid(int max_m, int nb)
 {
     BlockedPlan p;
     if (!plan_for(max_m, p)) return false;
+    if (nb == 64) return p.rows_per_thread <= 2;     // two-level: 64-wide blocks of two 32-wide panels
     return (nb == 8 || nb == 16 || nb == 32) && p.rows_per_thread * nb <= 64;
 }
 
@@ -570,6 +659,7 @@ static magma_int_t blocked_launch(const SizeArgs& sz, int max_m, int max_n, T** 
     MGI_LAUNCH_CHECK();
     const int max_minmn = std::min(max_m, max_n);
     switch (plan.nb) {
+        case 64: return blocked_steps2<T, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 32: return blocked_steps<T, 32, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 16: return blocked_steps<T, 16, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
         case 8:  return blocked_steps<T, 8, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);

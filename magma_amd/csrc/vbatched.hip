@@ -54,9 +54,13 @@ struct VBucket {
     int offset;      // into the concatenated map
 };
 
-enum VbVariant { kVbAuto = 0, kVbTile = 1, kVbBlocked32 = 2, kVbBlocked16 = 3, kVbBlocked8 = 4 };
+enum VbVariant { kVbAuto = 0, kVbTile = 1, kVbBlocked32 = 2, kVbBlocked16 = 3, kVbBlocked8 = 4, kVbBlocked2x32 = 5 };
 
-static int variant_nb(int v) { return v == kVbBlocked32 ? 32 : v == kVbBlocked16 ? 16 : v == kVbBlocked8 ? 8 : 0; }
+// panel width of a blocked variant (64 = the two-level form)
+static int variant_nb(int v)
+{
+    return v == kVbBlocked2x32 ? 64 : v == kVbBlocked32 ? 32 : v == kVbBlocked16 ? 16 : v == kVbBlocked8 ? 8 : 0;
+}
 
 // Group a batch (host copies of the sizes) into buckets. Matrices with a zero
 // dimension are left out (nothing to factor). Returns false if some size is

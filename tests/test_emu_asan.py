@@ -17,7 +17,7 @@ import build_emu  # noqa: E402
 
 SCENARIOS = ["lu", "overflow", "singular", "nonconv", "acc:fp32:ir:100", "acc:bf16:gmres:100",
              "acc:fp16:ir:10000", "acc:bf16x3:ir:100:256", "acc:fp16:gmres:100:177", "dhgesv:200:fp32:ir",
-             "dhgesv:257:bf16:gmres", "vbplan:3", "vbplan:2"]
+             "dhgesv:257:bf16:gmres", "vbplan:5", "vbplan:2"]
 
 
 @pytest.fixture(scope="module")
