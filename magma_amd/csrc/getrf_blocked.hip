@@ -268,8 +268,9 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
 // ---------------------------------------------------------------------------
 // 3. A12 := L11^{-1} A12 (unit lower). NB lanes run down the rows of one
 //    column (64/NB columns per wave instruction), so loads and stores are
-//    contiguous; the solved x_j reaches the rows below through a cross-lane
-//    read. Same operation order as a column-oriented dtrsm.
+//    contiguous; the solved x_j reaches the rows below through scalar lane
+//    reads (no LDS round trip in the dependent chain). Same operation order
+//    as a column-oriented dtrsm.
 // ---------------------------------------------------------------------------
 constexpr int kTrsmCols = 64;      // columns per workgroup
 
@@ -311,7 +312,13 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
         T x = on ? X[i] : T(0);
 #pragma unroll
         for (int j = 0; j < NB - 1; ++j) {
-            const T xj = shfl_t(x, cq * NB + j);
+            // x_j of this lane's column: one scalar read per column group
+            T xj = readlane_t(x, j);
+#pragma unroll
+            for (int g = 1; g < CPW; ++g) {
+                const T v = readlane_t(x, g * NB + j);
+                xj = cq == g ? v : xj;
+            }
             if (i > j) x = fma(-sL[i + j * NB], xj, x);
         }
         if (on) X[i] = x;
