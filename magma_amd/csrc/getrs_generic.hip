@@ -21,9 +21,9 @@
 namespace mgi {
 
 constexpr int kRsThreads = 256;
-constexpr int kRsCols = 8;     // right-hand sides per workgroup
+constexpr int kRsCols = 8;     // right-hand sides per workgroup (16 when the batch has more than 8)
 
-template <typename T>
+template <typename T, int NCOLS>
 __global__ void __launch_bounds__(kRsThreads)
 getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
                      T** __restrict__ dB_array)
@@ -32,12 +32,12 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     const int b = sz.map ? sz.map[blockIdx.x] : blockIdx.x;
     int n, nrhs, ldda, lddb;
     sz.get(b, n, nrhs, ldda, lddb);
-    const int c0 = blockIdx.y * kRsCols;
+    const int c0 = blockIdx.y * NCOLS;
     if (n <= 0 || c0 >= nrhs) return;
-    T* sB = reinterpret_cast<T*>(rs_raw);                       // n x kRsCols, ld = n
-    int* perm = reinterpret_cast<int*>(rs_raw + (size_t)sz.max_n * kRsCols * sizeof(T));
+    T* sB = reinterpret_cast<T*>(rs_raw);                       // n x NCOLS, ld = n
+    int* perm = reinterpret_cast<int*>(rs_raw + (size_t)sz.max_n * NCOLS * sizeof(T));
     const int tid = threadIdx.x;
-    const int nr = min(kRsCols, nrhs - c0);
+    const int nr = min(NCOLS, nrhs - c0);
     const T* A = dA_array[b];
     const int* ipiv = ipiv_array[b];
     T* B = dB_array[b] + (size_t)c0 * lddb;
@@ -93,19 +93,19 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
             __syncthreads();
             // rows below exist only after a full block
             for (int i = kb + kWave + tid; i < n; i += kRsThreads) {
-                T acc[kRsCols];
+                T acc[NCOLS];
 #pragma unroll
-                for (int c = 0; c < kRsCols; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
+                for (int c = 0; c < NCOLS; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
                 const T* Li = A + i + (size_t)kb * ldda;
 #pragma unroll 16
                 for (int j = 0; j < kWave; ++j) {
                     const T lij = Li[(size_t)j * ldda];
 #pragma unroll
-                    for (int c = 0; c < kRsCols; ++c)
+                    for (int c = 0; c < NCOLS; ++c)
                         if (c < nr) acc[c] = fma(-lij, sB[kb + j + c * n], acc[c]);
                 }
 #pragma unroll
-                for (int c = 0; c < kRsCols; ++c)
+                for (int c = 0; c < NCOLS; ++c)
                     if (c < nr) sB[i + c * n] = acc[c];
             }
             __syncthreads();
@@ -137,19 +137,19 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
             __syncthreads();
             // rows above exist only above a full block
             for (int i = tid; i < kb; i += kRsThreads) {
-                T acc[kRsCols];
+                T acc[NCOLS];
 #pragma unroll
-                for (int c = 0; c < kRsCols; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
+                for (int c = 0; c < NCOLS; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
                 const T* Ui = A + i + (size_t)kb * ldda;
 #pragma unroll 16
                 for (int j = kWave - 1; j >= 0; --j) {
                     const T uij = Ui[(size_t)j * ldda];
 #pragma unroll
-                    for (int c = 0; c < kRsCols; ++c)
+                    for (int c = 0; c < NCOLS; ++c)
                         if (c < nr) acc[c] = fma(-uij, sB[kb + j + c * n], acc[c]);
                 }
 #pragma unroll
-                for (int c = 0; c < kRsCols; ++c)
+                for (int c = 0; c < NCOLS; ++c)
                     if (c < nr) sB[i + c * n] = acc[c];
             }
             __syncthreads();
@@ -353,7 +353,8 @@ static magma_int_t getrs_wave_launch(const RsSizes& sz, int max_nrhs, T** dA_arr
     return MAGMA_SUCCESS;
 }
 
-size_t getrs_generic_bytes(int n, int elem) { return (size_t)n * kRsCols * elem + 2 * (size_t)n * sizeof(int); }
+static size_t generic_bytes(int n, int elem, int ncols) { return (size_t)n * ncols * elem + 2 * (size_t)n * sizeof(int); }
+size_t getrs_generic_bytes(int n, int elem) { return generic_bytes(n, elem, kRsCols); }
 size_t getrs_generic_bytes(int n) { return getrs_generic_bytes(n, sizeof(double)); }
 
 template <typename T>
@@ -368,12 +369,24 @@ magma_int_t getrs_generic_launch_t(int trans, const RsSizes& sz, int max_nrhs, T
         if constexpr (sizeof(T) == 4)
             return getrs_wave_launch<T, 8>(sz, max_nrhs, dA_array, ipiv_array, dB_array, batch, stream);
     }
-    const size_t lds = getrs_generic_bytes(sz.max_n, sizeof(T));
+    // more than 8 right-hand sides: 16 per workgroup, so each system's
+    // factors are streamed once for up to 16 columns
+    const size_t lds16 = generic_bytes(sz.max_n, sizeof(T), 2 * kRsCols);
+    if (max_nrhs > kRsCols && lds16 <= max_dynamic_lds()) {
+        MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T, 2 * kRsCols>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16));
+        dim3 grid(batch, ceildiv(max_nrhs, 2 * kRsCols));
+        getrs_generic_kernel<T, 2 * kRsCols><<<grid, kRsThreads, lds16, stream>>>(trans, sz, dA_array, ipiv_array,
+                                                                                dB_array);
+        MGI_LAUNCH_CHECK();
+        return MAGMA_SUCCESS;
+    }
+    const size_t lds = generic_bytes(sz.max_n, sizeof(T), kRsCols);
     if (lds > max_dynamic_lds()) return MAGMA_ERR_NOT_SUPPORTED;
-    MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T>,
+    MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T, kRsCols>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     dim3 grid(batch, ceildiv(max_nrhs, kRsCols));
-    getrs_generic_kernel<T><<<grid, kRsThreads, lds, stream>>>(trans, sz, dA_array, ipiv_array, dB_array);
+    getrs_generic_kernel<T, kRsCols><<<grid, kRsThreads, lds, stream>>>(trans, sz, dA_array, ipiv_array, dB_array);
     MGI_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }

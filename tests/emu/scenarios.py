@@ -306,7 +306,7 @@ def scenario_getrs(q, n, nrhs, trans, batch=2):
     check(rc == 0 and be < 1e-14, f"dgetrs {n} {nrhs} {trans}")
 
 
-def scenario_vbplan(q, variant, ns=(129, 140, 200, 256, 177, 64, 30, 300), rs=(1, 3, 16, 2, 5, 4, 1, 2)):
+def scenario_vbplan(q, variant, ns=(129, 140, 200, 256, 177, 64, 30, 300, 290), rs=(1, 3, 16, 2, 5, 4, 1, 12, 3)):
     """Variable-size plan executed with one kernel choice for every bucket
     (bucket-invalid choices fall back to the built-in rule)."""
     rng = np.random.default_rng(variant * 13 + len(ns))
