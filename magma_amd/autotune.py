@@ -1,10 +1,10 @@
 """Autotuner for variable-size batches: picks, per n-bucket, the kernel
 variant of the vbatched plan (register-tiled LU for n <= 128, or the blocked
 LU with panel width 32, 16 or 8, or two 32-wide panels per 64-wide block) by
-timing each valid variant on the bucket's
-own systems, replayed as a hipGraph. Results are cached in a JSON file keyed
-by device name and bucket id; `load_variants` reads the cache and tunes only
-buckets that are missing from it.
+timing each valid variant on the bucket's own systems, replayed as a hipGraph.
+Results are cached in a JSON file keyed by device (architecture and CU count)
+and bucket id; `load_variants` reads the cache and tunes only buckets that are
+missing from it.
 
 The bucket boundaries follow the plan's (magma_amd/csrc/vbatched.hip):
 n <= 128 in steps of 16, then 64 up to 512, 128 up to 1024, 256 up to 2048.
@@ -18,7 +18,7 @@ from pathlib import Path
 import numpy as np
 import torch
 
-from .batched import IpivV, VBatch, VbatchedPlan, default_queue
+from .batched import IpivV, VBatch, VbatchedPlan
 
 VARIANT_NAMES = {1: "tile", 2: "blocked nb=32", 3: "blocked nb=16", 4: "blocked nb=8", 5: "blocked 2x32"}
 CACHE = Path(os.environ.get("MAGMA_AMD_TUNING", Path(__file__).resolve().parent / "tuning" / "vbatched.json"))
