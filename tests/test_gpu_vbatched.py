@@ -163,7 +163,7 @@ def test_gesv_vbatched(mb, oracle):
             assert np.array_equal(ipiv.get(i, n), ipo[0]), n
 
 
-@pytest.mark.parametrize("variant", [None, 2, 3, 4])
+@pytest.mark.parametrize("variant", [None, 2, 3, 4, 5])
 def test_vbatched_plan_graph_replay(mb, oracle, variant):
     """A plan executed directly, then captured into a hipGraph and replayed
     several times with eager work in between; every replay matches."""
@@ -216,3 +216,25 @@ def test_sgesv_batched(mb, oracle, n, nrhs):
     LUo, ipo, _ = oracle.dgetrf(A.astype(np.float64))
     tf = margin > 1e-4          # FP32 rounding: compare pivots where the gap is clear
     assert np.array_equal(ipiv.cpu().numpy()[tf], ipo[tf])
+
+
+@pytest.mark.parametrize("variant", [2, 5])
+def test_vbatched_plan_lu_parity(mb, oracle, variant):
+    """The blocked variants (32-wide panels, and two 32-wide panels per
+    64-wide block) give the oracle's pivots on tie-free systems and its LU
+    within the forward-error bound, including orders that end in a partial
+    block (n = 129, 177, 300, 449)."""
+    rng = np.random.default_rng(40 + variant)
+    ns = [129, 140, 177, 192, 200, 256, 257, 300, 449, 512]
+    rs = [1] * len(ns)
+    A, B, ref = _random_system_batch(mb, rng, ns, rs)
+    plan = mb.VbatchedPlan(ns, rs, {b: variant for b in range(22)})
+    ipiv = mb.IpivV(ns)
+    info = torch.zeros(len(ns), dtype=torch.int32, device="cuda")
+    plan.execute(A, B, ipiv, info)
+    torch.cuda.synchronize()
+    plan.close()
+    assert (info.cpu().numpy() == 0).all()
+    for i, n in enumerate(ns):
+        Am, _ = ref[i]
+        check_one(oracle, Am, A.get(i), ipiv.get(i, n), 0, n, n, f"variant {variant} n={n}")
