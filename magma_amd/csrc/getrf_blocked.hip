@@ -526,7 +526,7 @@ struct BlockedPlan {
 static bool plan_for(int max_m, BlockedPlan& p)
 {
     if (max_m <= 256)       { p.rows_per_thread = 1; p.nb = 32; }
-    else if (max_m <= 512)  { p.rows_per_thread = 2; p.nb = 32; }
+    else if (max_m <= 512)  { p.rows_per_thread = 2; p.nb = 64; }   // two-level: 2 x 32 per 64-wide block
     else if (max_m <= 1024) { p.rows_per_thread = 4; p.nb = 16; }
     else if (max_m <= 2048) { p.rows_per_thread = 8; p.nb = 8; }
     else return false;
