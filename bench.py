@@ -75,12 +75,38 @@ def parse():
                     help="also time BF16-LU + FP64 refinement against FP64 dgesv on 2,000 x 256^2 (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1,
                     help="also time the differentiable solve (forward + backward) on 8,192 x 64^2 (0 = skip)")
+    ap.add_argument("--dense", type=int, default=1,
+                    help="also run the dense mixed-precision leg (one system of order 24,576 at three condition "
+                         "numbers); 0 skips")
+    ap.add_argument("--dense-n", type=int, default=24576)
+    ap.add_argument("--c1", type=int, default=1,
+                    help="also run the small CPU-path leg (1,000 x 32^2, host LAPACK beside the GPU); 0 skips")
     ap.add_argument("--vbatched", type=int, default=1,
                     help="also run the variable-size leg (50,000 systems, n ~ U[8,512], nrhs ~ U[1,16]); 0 skips")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the workload, copy a buffer of this many MiB once on the device (a kernel of "
                          "known traffic that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     return ap.parse_args()
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Threads for the host LAPACK baseline: the cores this process may use,
+    capped at 16 (the host share of one GPU on the measurement boxes)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
 
 
 def cpu_baseline(A_host, B_host, n, nrhs, min_secs=10.0):
@@ -92,11 +118,7 @@ def cpu_baseline(A_host, B_host, n, nrhs, min_secs=10.0):
     lib = oracle.host_lapack()
     if lib is None:
         return None
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = cpu_threads()
     # Repeat passes over the sample until about min_secs of wall time (a few
     # seconds of every core's work), so the rate is not a sub-second blip.
     secs, passes, X = 0.0, 0, None
@@ -117,7 +139,42 @@ def cpu_baseline(A_host, B_host, n, nrhs, min_secs=10.0):
                    f"threads, LAPACK sequential ({lib.cb_path().decode()}); {secs:.2f} s wall in total, "
                    f"max tester backward error {float(np.nanmax(be)):.2e}"),
         "gflops": solved * (flops_getrf(n) + flops_getrs(n, nrhs)) / secs / 1e9,
+        "cpu_model": cpu_model(),
+        "host_cores_visible": os.cpu_count(),
     }
+
+
+def c1_leg(torch, mb, lib, queue, dev, seed, batch=1000, n=32, nrhs=1, steps=20):
+    """The reference's CPU path at small size (lapackf77_dgesv looped over the
+    batch, testing_zgesv_batched.cpp:158-180) on 1,000 systems 32x32 with one
+    RHS, timed on the host cores, with the GPU's fused tile LU + solve of the
+    same batch beside it (inputs resident, HIP events on the queue stream)."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    A = torch.rand((batch, n, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+    B = torch.rand((batch, nrhs, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+    A_host, B_host = A.cpu().numpy(), B.cpu().numpy()
+    sets = [(A.clone(), B.clone()) for _ in range(steps + 1)]
+    mb.dgesv_batched(*sets[0])
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for k in range(1, steps + 1):
+        mb.dgesv_batched(*sets[k])
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / steps
+    X = sets[1][1]
+    be = float(backward_errors(torch, A, X, B).max())
+    nbytes = batch * algorithmic_bytes(n, nrhs)
+    cpu = cpu_baseline(A_host, B_host, n, nrhs, min_secs=5.0)
+    return {"workload": f"dgesv {batch} x {n}x{n} FP64, nrhs={nrhs} (the reference's CPU path)",
+            "gpu": {"solves_per_s": round(batch / t, 1), "us_per_call": round(t * 1e6, 2),
+                    "backward_error_max": be,
+                    "roofline": {"bound": "hbm", "achieved": round(nbytes / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+                                 "algorithmic_bytes_per_launch": nbytes}},
+            "cpu": cpu}
 
 
 def svd_geo_batch(torch, batch, n, cond, gen, dev):
@@ -148,7 +205,15 @@ def backward_errors(torch, A_cm, X_cm, B_cm):
     return rn / (an * xn)
 
 
-def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n=256, nrhs=1, cond=1e2):
+def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n=256, nrhs=1,
+                        conds=(1e2, 1e4, 1e6)):
+    """The batch half of the mixed-precision workload at each condition
+    number (one run per cond)."""
+    return {f"cond_{c:.0e}": mixed_precision_batch(torch, mb, lib, queue, dev, seed, steps, batch, n, nrhs, c)
+            for c in conds}
+
+
+def mixed_precision_batch(torch, mb, lib, queue, dev, seed, steps, batch, n, nrhs, cond):
     """BF16- and FP16-update LU + FP64 refinement (classical and FGMRES)
     against FP64 dgesv_batched on the same batch of svd_geo systems: solves/s,
     FP64-credited GFLOP/s, refinement steps and the worst backward / forward
@@ -199,6 +264,95 @@ def mixed_precision_leg(torch, mb, lib, queue, dev, seed, steps=3, batch=2000, n
     return res
 
 
+def haar_cols(torch, n, gen, dev):
+    """Haar-distributed orthogonal n x n matrix: QR of a Gaussian matrix with
+    the sign of R's diagonal moved into Q."""
+    Z = torch.randn(n, n, dtype=torch.float64, device=dev, generator=gen)
+    Q, R = torch.linalg.qr(Z)
+    Q *= torch.sign(torch.diagonal(R))[None, :]
+    del Z, R
+    return Q
+
+
+def dense_leg(torch, dev, seed, n=24576, conds=(1e2, 1e4, 1e6),
+              variants=(("bf16", "ir"), ("bf16", "gmres"), ("fp16", "ir"), ("fp16", "gmres"))):
+    """The dense half of the mixed-precision workload: one system of order n,
+    A = U diag(sigma) V^T (Haar U and V, generated once; sigma geometric to
+    cond), b ~ U[-1,1]. FP64 magma_dgetrf_gpu + magma_dgetrs_gpu against the
+    refinement solver magma_dxgesv_gmres_gpu with BF16 / FP16 updates and
+    classical or GMRES refinement; each call is timed whole (host clock,
+    device synchronised), as a caller sees it. The low-precision LU alone is
+    timed through magma_sgetrf_prec_gpu for its MFMA fraction."""
+    from magma_amd import dense, iteref
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    t_gen = time.perf_counter()
+    U = haar_cols(torch, n, gen, dev)
+    V = haar_cols(torch, n, gen, dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    flop = flops_getrf(n) + flops_getrs(n, 1)
+    out = {"workload": f"one FP64 system n={n}, svd_geo (Haar U, V), nrhs=1", "generation_s": round(t_gen, 2),
+           "fp64_credited_gflop": round(flop / 1e9, 1)}
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    # low-precision LU alone (MFMA fraction); the matrix does not matter for its speed
+    Af = (U * 0.5).T.contiguous().float()
+    lu_only = {}
+    for facto, peak in (("bf16", 2500.0), ("fp16", 2500.0)):
+        W = Af.clone()
+        (ipiv, info), t = timed(lambda: dense.sgetrf_prec_gpu(W, facto))
+        lu_only[facto] = {"ms": round(t * 1e3, 2), "tflops": round(flops_getrf(n) / t / 1e12, 2),
+                          "frac_of_16bit_mfma_peak": round(flops_getrf(n) / t / 1e12 / peak, 4)}
+        del W
+    del Af
+    out["low_precision_lu"] = lu_only
+    for cond in conds:
+        sig = cond ** (-torch.arange(n, dtype=torch.float64, device=dev) / max(n - 1, 1))
+        A = ((U * sig[None, :]) @ V.T.contiguous()).T.contiguous()       # column storage of U S V^T
+        b = torch.rand((1, n), dtype=torch.float64, device=dev, generator=gen) * 2 - 1
+        LU = A.clone()
+        (ipiv, info), t_lu = timed(lambda: dense.getrf_gpu(LU))
+        X64 = b.clone()
+        _, t_rs = timed(lambda: dense.getrs_gpu(LU, ipiv, X64))
+        del LU
+        t64 = t_lu + t_rs
+        rec = {"fp64": {"ms": round(t64 * 1e3, 2), "getrf_ms": round(t_lu * 1e3, 2),
+                        "getrs_ms": round(t_rs * 1e3, 2), "solves_per_s": round(1 / t64, 3),
+                        "tflops": round(flop / t64 / 1e12, 2),
+                        "getrf_fp64_frac": round(flops_getrf(n) / t_lu / 1e12 / FP64_PEAK_TFLOPS, 4),
+                        "backward_error": dense_be(torch, A, X64, b)}}
+        for facto, solver in variants:
+            Ac = A.clone()
+            (X, _, _, it, info), t = timed(lambda: iteref.dhgesv_iteref_gpu(Ac, b.clone(), facto=facto,
+                                                                            solver=solver))
+            del Ac
+            fe = float((X - X64).abs().max() / X64.abs().max())
+            rec[f"{facto}_{solver}"] = {"ms": round(t * 1e3, 2), "solves_per_s": round(1 / t, 3),
+                                        "tflops_fp64_credited": round(flop / t / 1e12, 2), "iter": it,
+                                        "info": info, "backward_error": dense_be(torch, A, X, b),
+                                        "forward_error_vs_fp64": fe, "speedup_vs_fp64": round(t64 / t, 3)}
+        out[f"cond_{cond:.0e}"] = rec
+        del A
+        torch.cuda.empty_cache()
+    del U, V
+    torch.cuda.empty_cache()
+    return out
+
+
+def dense_be(torch, A_cols, X, B):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf), FP64 on the device."""
+    A = A_cols.T
+    R = B.T - A @ X.T
+    return float((R.abs().amax(0) / (A.abs().sum(1).max() * X.T.abs().amax(0))).max())
+
+
 def adjoint_leg(torch, dev, seed, steps=5, batch=8192, n=64, nrhs=8):
     """Forward + backward of the differentiable solve, FP64 and FP32."""
     from magma_amd.autograd import BatchedSolve
@@ -225,7 +379,14 @@ def adjoint_leg(torch, dev, seed, steps=5, batch=8192, n=64, nrhs=8):
             X, ga, gb = step()
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / steps
-        res[name] = {"systems_per_s": round(batch / t, 1), "ms_fwd_bwd": round(t * 1e3, 3)}
+        e = 8 if dt == torch.float64 else 4
+        # forward: A read + LU written, B read + X written, ipiv; backward:
+        # LU read, G read + grad_B written + X read, grad_A written
+        nbytes = batch * (2 * e * n * n + 2 * e * n * nrhs + 4 * n + e * n * n + 3 * e * n * nrhs + e * n * n)
+        res[name] = {"systems_per_s": round(batch / t, 1), "ms_fwd_bwd": round(t * 1e3, 3),
+                     "roofline": {"bound": "hbm", "achieved": round(nbytes / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes_per_step": nbytes}}
     return res
 
 
@@ -443,6 +604,15 @@ def main():
         print("[bench] adjoint leg", file=sys.stderr, flush=True)
         adjoint = adjoint_leg(torch, dev, args.seed)
 
+    dense_res = None
+    if args.dense:
+        print("[bench] dense mixed-precision leg", file=sys.stderr, flush=True)
+        dense_res = dense_leg(torch, dev, args.seed, n=args.dense_n)
+    c1 = None
+    if args.c1:
+        print("[bench] small CPU-path leg", file=sys.stderr, flush=True)
+        c1 = c1_leg(torch, mb, lib, queue, dev, args.seed)
+
     vb = None
     if args.vbatched:
         print("[bench] variable-size leg", file=sys.stderr, flush=True)
@@ -475,6 +645,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "dense_mixed_precision": dense_res,
+        "small_cpu_path": c1,
         "adjoint": adjoint,
         "vbatched": vb,
     }

@@ -40,6 +40,7 @@ typedef struct magma_queue* magma_queue_t;
 #define MAGMA_ERR                  -100
 #define MAGMA_ERR_NOT_INITIALIZED  -101
 #define MAGMA_ERR_NOT_SUPPORTED    -103
+#define MAGMA_ERR_ILLEGAL_VALUE    -104
 #define MAGMA_ERR_HOST_ALLOC       -112
 #define MAGMA_ERR_DEVICE_ALLOC     -113
 #define MAGMA_ERR_INVALID_PTR      -115
@@ -68,8 +69,35 @@ typedef enum {
     Magma_REFINE_IRGMSTRS = 922,
     Magma_REFINE_IRGMDTRS = 923,
     Magma_REFINE_GMSTRS   = 924,   /* FGMRES preconditioned by the low-prec. LU    */
-    Magma_REFINE_GMDTRS   = 925
+    Magma_REFINE_GMDTRS   = 925,
+    Magma_REFINE_GMGMSTRS = 926,
+    Magma_REFINE_GMGMDTRS = 927
 } magma_refinement_t;
+
+/* Mixed-precision GEMM selectors of magma_xshgetrf_gpu
+ * (include/magma_types.h:885-908), same values. */
+typedef enum {
+    Magma_MP_BASE_SS              = 950,
+    Magma_MP_BASE_DD              = 951,
+    Magma_MP_BASE_XHS             = 952,
+    Magma_MP_BASE_XSH             = 953,
+    Magma_MP_BASE_XHD             = 954,
+    Magma_MP_BASE_XDH             = 955,
+    Magma_MP_ENABLE_DFLT_MATH     = 960,
+    Magma_MP_ENABLE_TC_MATH       = 961,
+    Magma_MP_SGEMM                = 962,
+    Magma_MP_HGEMM                = 963,
+    Magma_MP_GEMEX_I32_O32_C32    = 964,
+    Magma_MP_GEMEX_I16_O32_C32    = 965,
+    Magma_MP_GEMEX_I16_O16_C32    = 966,
+    Magma_MP_GEMEX_I16_O16_C16    = 967,
+    Magma_MP_TC_SGEMM             = 968,
+    Magma_MP_TC_HGEMM             = 969,
+    Magma_MP_TC_GEMEX_I32_O32_C32 = 970,
+    Magma_MP_TC_GEMEX_I16_O32_C32 = 971,
+    Magma_MP_TC_GEMEX_I16_O16_C32 = 972,
+    Magma_MP_TC_GEMEX_I16_O16_C16 = 973
+} magma_mp_type_t;
 
 /* ------------------------------------------------------------------------- */
 /* Runtime (include/magma_auxiliary.h, include/magma_copy.h)                  */
@@ -242,6 +270,43 @@ magma_int_t magma_sgesv_batched(magma_int_t n, magma_int_t nrhs, float **dA_arra
 void magma_sset_pointer(float **output_array, float *input, magma_int_t lda,
                         magma_int_t row, magma_int_t column, magma_int_t batch_offset,
                         magma_int_t batchCount, magma_queue_t queue);
+
+/* ------------------------------------------------------------------------- */
+/* One large matrix on the whole device (dense LU)                            */
+/* ------------------------------------------------------------------------- */
+/* include/magma_z.h:612 (src/zgetrf_gpu.cpp:542), d form: LU with partial
+ * pivoting of the m x n matrix dA in place; ipiv is host memory (min(m,n)
+ * LAPACK interchanges, 1-based); info > 0: U(info,info) is exactly zero.
+ * Host-synchronous. */
+magma_int_t magma_dgetrf_gpu(magma_int_t m, magma_int_t n, magmaDouble_ptr dA, magma_int_t ldda,
+                             magma_int_t *ipiv, magma_int_t *info);
+/* include/magma_z.h:638 (src/zgetrf_gpu.cpp:607): the all-GPU variant; same
+ * arguments and results as magma_dgetrf_gpu. */
+magma_int_t magma_dgetrf_native(magma_int_t m, magma_int_t n, magmaDouble_ptr dA, magma_int_t ldda,
+                                magma_int_t *ipiv, magma_int_t *info);
+/* include/magma_z.h:713 (src/zgetrs_gpu.cpp:292), d form: solve op(A) X = B
+ * with the factors of magma_dgetrf_gpu (trans as LAPACK dgetrs). */
+magma_int_t magma_dgetrs_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                             magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv,
+                             magmaDouble_ptr dB, magma_int_t lddb, magma_int_t *info);
+/* s forms of include/magma_z.h:612 and :713 (include/magma_s.h). */
+magma_int_t magma_sgetrf_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr dA, magma_int_t ldda,
+                             magma_int_t *ipiv, magma_int_t *info);
+magma_int_t magma_sgetrs_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                             magmaFloat_ptr dA, magma_int_t ldda, magma_int_t *ipiv,
+                             magmaFloat_ptr dB, magma_int_t lddb, magma_int_t *info);
+
+/* include/magma_htc.h:119 (src/xshgetrf_gpu.cpp:86): FP32 LU with FP16-input,
+ * FP32-accumulate trailing updates on the matrix cores when enable_tc is
+ * Magma_MP_ENABLE_TC_MATH (FP32 updates for Magma_MP_ENABLE_DFLT_MATH or an
+ * FP32 mp_algo_type). ipiv is host memory. Host-synchronous. */
+magma_int_t magma_xshgetrf_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr dA, magma_int_t ldda,
+                               magma_int_t *ipiv, magma_int_t *info,
+                               magma_mp_type_t enable_tc, magma_mp_type_t mp_algo_type);
+/* No MAGMA counterpart: the FP32 LU with the update format of a refinement
+ * factorisation type (Magma_PREC_SS, _SHT, _SBT, _SB3T); -7 for another. */
+magma_int_t magma_sgetrf_prec_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr dA, magma_int_t ldda,
+                                  magma_int_t *ipiv, magma_int_t *info, magma_refinement_t facto_type);
 
 /* ------------------------------------------------------------------------- */
 /* Mixed-precision iterative refinement                                        */

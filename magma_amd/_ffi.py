@@ -25,6 +25,8 @@ MagmaNoTrans, MagmaTrans, MagmaConjTrans = 111, 112, 113
 Magma_PREC_SS, Magma_PREC_SHT, Magma_PREC_XSH_163232TC = 900, 905, 917
 Magma_PREC_SBT, Magma_PREC_SB3T = 940, 941
 Magma_REFINE_IRSTRS, Magma_REFINE_GMSTRS = 920, 924
+Magma_MP_ENABLE_DFLT_MATH, Magma_MP_ENABLE_TC_MATH = 960, 961
+Magma_MP_SGEMM, Magma_MP_GEMEX_I16_O32_C32 = 962, 965
 MAGMA_ERR_NOT_SUPPORTED = -103
 
 # name -> (restype, argtypes); pointers to device memory are passed as c_void_p.
@@ -97,6 +99,13 @@ _PROTOS = {
     "magma_dxgesv_gmres_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                        c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, i32p, i32p,
                                        ctypes.POINTER(c_double)]),
+    "magma_dgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),
+    "magma_dgetrf_native": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),
+    "magma_sgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),
+    "magma_dgetrs_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_int, i32p]),
+    "magma_xshgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p, c_int, c_int]),
+    "magma_sgetrf_prec_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p, c_int]),
+    "magma_sgetrs_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_int, i32p]),
     "magma_dxgesv_iteref_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
                                             c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
 }

@@ -39,8 +39,13 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libmagma_mi355x.so")
 
 
-def _compile(src: Path, verbose: bool) -> Path:
+def _compile(src: Path, verbose: bool, incremental: bool = False) -> Path:
     obj = BUILD / (src.stem + ".o")
+    if incremental and obj.exists():
+        newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.h")] +
+                     [(PKG.parent / "include" / "magma_mi355x.h").stat().st_mtime])
+        if obj.stat().st_mtime > newest:
+            return obj
     cmd = [hipcc(), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -52,14 +57,14 @@ def _compile(src: Path, verbose: bool) -> Path:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> Path:
+def build(jobs: int | None = None, verbose: bool = False, incremental: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
     if not sources:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     jobs = jobs or min(len(sources), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), sources))
+        objs = list(ex.map(lambda s: _compile(s, verbose, incremental), sources))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     if verbose:
@@ -75,8 +80,10 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="skip sources whose object is newer than the source and every header")
     args = ap.parse_args()
-    print(build(args.jobs, args.verbose))
+    print(build(args.jobs, args.verbose, args.incremental))
 
 
 if __name__ == "__main__":

@@ -665,13 +665,24 @@ static magma_int_t blocked_launch(const SizeArgs& sz, int max_m, int max_n, T** 
     fill_int_kernel<<<ceildiv(batch, 256), 256, 0, stream>>>(info_array, 0, batch, sz.map);
     MGI_LAUNCH_CHECK();
     const int max_minmn = std::min(max_m, max_n);
-    switch (plan.nb) {
-        case 64: return blocked_steps2<T, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        case 32: return blocked_steps<T, 32, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        case 16: return blocked_steps<T, 16, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        case 8:  return blocked_steps<T, 8, MODE>(plan.rows_per_thread, sz, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, batch, stream);
-        default: return MAGMA_ERR_NOT_SUPPORTED;
+    // the row-interchange, triangular-solve and update kernels put the batch
+    // in grid y or z: split larger batches (src/zgetrf_batched.cpp splits at
+    // the queue's maximum batch the same way)
+    for (int c0 = 0; c0 < batch; c0 += kMaxGridYZ) {
+        SizeArgs part = sz;
+        part.base = sz.base + c0;
+        const int cb = std::min(kMaxGridYZ, batch - c0);
+        magma_int_t rc;
+        switch (plan.nb) {
+            case 64: rc = blocked_steps2<T, MODE>(plan.rows_per_thread, part, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, cb, stream); break;
+            case 32: rc = blocked_steps<T, 32, MODE>(plan.rows_per_thread, part, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, cb, stream); break;
+            case 16: rc = blocked_steps<T, 16, MODE>(plan.rows_per_thread, part, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, cb, stream); break;
+            case 8:  rc = blocked_steps<T, 8, MODE>(plan.rows_per_thread, part, max_m, max_n, max_minmn, dA_array, ipiv_array, info_array, cb, stream); break;
+            default: rc = MAGMA_ERR_NOT_SUPPORTED; break;
+        }
+        if (rc != MAGMA_SUCCESS) return rc;
     }
+    return MAGMA_SUCCESS;
 }
 
 magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const int* n_arr, const int* ld_arr,

@@ -1,0 +1,860 @@
+// Blocked right-looking LU with partial pivoting of one large column-major
+// matrix on the whole GPU (MAGMA's magma_dgetrf_gpu / magma_dgetrf_native,
+// src/zgetrf_gpu.cpp:213-452 and :542-663, and the FP32 LU with low-precision
+// trailing updates of src/xshgetrf_gpu.cpp:86-419).
+//
+// Blocks of NB columns, each factored as two panels of PW columns:
+//   panel    dense_panel_kernel: the (m-k) x PW panel is spread over G
+//            workgroups, one row per thread in registers. Per column every
+//            workgroup publishes its best candidate row; after all G have
+//            published, each workgroup picks the same pivot (largest |a|,
+//            lowest LAPACK position, NaN only at the first position), and
+//            applies the rank-1 update to its rows. Rows are not moved while
+//            the panel runs: each row carries its LAPACK position, and the
+//            final rows are written to their positions at the end.
+//   swaps    dense_pivmap_kernel composes a block's interchanges into
+//            (source, destination) row pairs; dense_laswp_kernel moves those
+//            rows in every other column, one wave per column.
+//   U12      L11^{-1} (dense_trtri_kernel) times A12 on MFMA, in the storage
+//            precision (FP64, or FP32 as MAGMA's strsm).
+//   update   A22 -= A21 U12 on MFMA (dense_gemm_kernel, 128 x 128 tiles):
+//            FP64; or FP32 storage with FP32, BF16, FP16 or split-BF16
+//            inputs and FP32 accumulation.
+// Inside a block, the first panel's interchanges, U12 and update are applied
+// to the block's second half before the second panel; the whole block's
+// interchanges, U12 and update are then applied once to the rest.
+#include "magma_internal.h"
+#include "lu_device.h"
+#include "dense_kernels.h"
+
+#include <algorithm>
+#include <climits>
+#include <type_traits>
+#include <utility>
+
+namespace mgi {
+
+namespace {
+
+template <int... Js, typename F>
+__device__ __forceinline__ void dense_for(std::integer_sequence<int, Js...>, F&& f)
+{
+    (f(std::integral_constant<int, Js>{}), ...);
+}
+
+template <typename T> struct DLim;
+template <> struct DLim<double> { static constexpr double sfmin = kSfminD; };
+template <> struct DLim<float>  { static constexpr float sfmin = kSfminF; };
+
+__device__ __forceinline__ void argmax_t(double& v, int& idx) { wave_argmax(v, idx); }
+__device__ __forceinline__ void argmax_t(float& v, int& idx) { wave_argmax_f(v, idx); }
+
+// A wait gives up after this many ticks of the 100 MHz constant clock (2 s)
+// and raises the error word instead of spinning on.
+constexpr long long kDenseWaitTicks = 200000000LL;
+
+__device__ __forceinline__ int ld_agent(const int* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Panel: G workgroups of 256 threads; thread t of workgroup g holds row
+// k + 256 g + t (columns k .. k+jb-1) in registers.
+// ---------------------------------------------------------------------------
+template <typename T, int PW>
+__global__ void __launch_bounds__(kDensePanelRows)
+dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __restrict__ ipiv, int* __restrict__ info,
+                   DenseWork w)
+{
+    constexpr int kW = kDensePanelRows / kWave;
+    __shared__ T s_wv[2][kW];       // per wave: best |a| and its position (parity-buffered)
+    __shared__ int s_wp[2][kW];
+    __shared__ T s_u[2][PW];        // pivot row
+    __shared__ int s_pp[2];         // pivot position
+
+    const int G = gridDim.x;
+    const int g = blockIdx.x;
+    const int t = threadIdx.x;
+    const int wv = t / kWave, ln = t % kWave;
+    const int mloc = m - k;
+    const int r = g * kDensePanelRows + t;
+    const bool valid = r < mloc;
+    int pos = k + r;                 // LAPACK position of the row this thread holds
+    bool active = valid;             // not yet chosen as a pivot
+    T* P = A + k + (size_t)k * lda;
+    const int gmax = w.gmax;
+    T* crow = reinterpret_cast<T*>(w.crow);
+
+    T a[PW];
+#pragma unroll
+    for (int c = 0; c < PW; ++c) a[c] = (valid && c < jb) ? P[r + (size_t)c * lda] : T(0);
+
+    dense_for(std::make_integer_sequence<int, PW>{}, [&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        constexpr int par = j & 1;
+        if (j < jb) {
+            const int diag = k + j;
+            // 1. best candidate of each wave, then of the workgroup
+            T v = T(-1);
+            int vp = INT_MAX;
+            if (active) {
+                T x = fabs(a[j]);
+                if (x != x) x = (pos == diag) ? T(INFINITY) : T(-1);
+                if (x >= T(0)) { v = x; vp = pos; }
+            }
+            argmax_t(v, vp);
+            if (ln == 0) { s_wv[par][wv] = v; s_wp[par][wv] = vp; }
+            __syncthreads();
+            T bv = s_wv[par][0];
+            int bp = s_wp[par][0];
+#pragma unroll
+            for (int q = 1; q < kW; ++q) {
+                const T ov = s_wv[par][q];
+                const int op = s_wp[par][q];
+                if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
+            }
+            // 2. publish: the holder of the workgroup's best row writes it,
+            //    its position, then the step tag (release)
+            const bool has = bp != INT_MAX;
+            if (has ? (active && pos == bp) : (t == 0)) {
+                const size_t slot = (size_t)par * gmax + g;
+                if (has) {
+                    T* dst = crow + slot * PW;
+#pragma unroll
+                    for (int c = j; c < PW; ++c) dst[c] = a[c];
+                }
+                w.cpos[slot] = has ? bp : -1;
+                __hip_atomic_store(&w.tag[slot], diag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            // 3. wave 0 waits for all G candidates, picks the pivot, stages
+            //    its row in LDS
+            if (wv == 0) {
+                const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                for (;;) {
+                    bool mine = true;
+                    for (int q = ln; q < G; q += kWave) mine = mine && (ld_agent(&w.tag[par * gmax + q]) == diag);
+                    if (__ballot(!mine) == 0ull) break;
+                    if (ld_agent(w.err) != 0) break;
+                    if ((long long)(__builtin_amdgcn_s_memrealtime() - t0) > kDenseWaitTicks) {
+                        if (ln == 0) __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                T cv = T(-1);
+                int cp = INT_MAX, cg = 0;
+                for (int q = ln; q < G; q += kWave) {
+                    const size_t slot = (size_t)par * gmax + q;
+                    const int p = w.cpos[slot];
+                    if (p >= 0) {
+                        T x = fabs(crow[slot * PW + j]);
+                        if (x != x) x = (p == diag) ? T(INFINITY) : T(-1);
+                        if (x >= T(0) && (x > cv || (x == cv && p < cp))) { cv = x; cp = p; cg = q; }
+                    }
+                }
+                const int mycp = cp;
+                argmax_t(cv, cp);
+                const unsigned long long hit = __ballot(mycp == cp && cp != INT_MAX);
+                int gw = 0;
+                if (hit != 0ull) gw = __shfl(cg, __ffsll((long long)hit) - 1, kWave);
+                // only a timed-out wait can leave no candidate; keep the
+                // position in range then
+                int pp = cp;
+                if (pp < diag || pp >= m) pp = diag;
+                const T* src = crow + ((size_t)par * gmax + gw) * PW;
+                for (int c = ln; c < PW; c += kWave) s_u[par][c] = (hit != 0ull) ? src[c] : T(0);
+                if (ln == 0) s_pp[par] = pp;
+            }
+            __syncthreads();
+            // 4. interchange bookkeeping, scaling and rank-1 update
+            const int pp = s_pp[par];
+            const T piv = s_u[par][j];
+            if (g == 0 && t == 0) {
+                ipiv[diag] = pp + 1;
+                if (piv == T(0)) atomicCAS(info, 0, diag + 1);
+            }
+            const bool win = active && pos == pp;
+            if (active && pos == diag) pos = pp;
+            if (win) { pos = diag; active = false; }
+            if (active && piv != T(0)) {
+                T l;
+                if (fabs(piv) >= DLim<T>::sfmin) l = a[j] * (T(1) / piv);
+                else l = a[j] / piv;
+                a[j] = l;
+#pragma unroll
+                for (int c = j + 1; c < PW; ++c) a[c] = fma(-l, s_u[par][c], a[c]);
+            }
+        }
+    });
+    // rows go to their final positions (all reads of the panel happened
+    // before the first wait, every write happens after the last one)
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < PW; ++c)
+            if (c < jb) A[pos + (size_t)(k + c) * lda] = a[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row pairs of the interchanges ipiv[s .. s+cnt) (LAPACK, 1-based, applied in
+// order): destination d receives the row found by tracing d back through
+// the interchanges. Destinations are the rows s .. s+cnt-1 and every
+// interchange partner below them (counted once). cnt <= 256.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(512)
+dense_pivmap_kernel(const int* __restrict__ ipiv, int s, int cnt, int* __restrict__ pairs, int* __restrict__ npairs,
+                    int maxp)
+{
+    __shared__ int piv[256];
+    __shared__ int count;
+    const int t = threadIdx.x;
+    if (t == 0) count = 0;
+    for (int i = t; i < cnt; i += blockDim.x) piv[i] = ipiv[s + i] - 1;
+    __syncthreads();
+    int d = -1;
+    if (t < cnt) {
+        d = s + t;
+    } else if (t < 2 * cnt) {
+        const int i = t - cnt;
+        const int p = piv[i];
+        if (p >= s + cnt) {
+            bool first = true;
+            for (int q = 0; q < i; ++q)
+                if (piv[q] == p) { first = false; break; }
+            if (first) d = p;
+        }
+    }
+    if (d >= 0) {
+        int row = d;
+        for (int i = cnt - 1; i >= 0; --i) {
+            const int p = piv[i];
+            if (row == s + i) row = p;
+            else if (row == p) row = s + i;
+        }
+        if (row != d) {
+            const int idx = atomicAdd(&count, 1);
+            if (idx < maxp) {
+                pairs[idx] = row;
+                pairs[maxp + idx] = d;
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) *npairs = min(count, maxp);
+}
+
+// Row moves of one block in the columns [c0, c1) and [c2, c3): one wave per
+// column reads every source row into registers; after the workgroup barrier
+// (which waits for every load) it writes every destination row.
+template <typename T, int MAXPL>
+__global__ void __launch_bounds__(256)
+dense_laswp_kernel(int m, T* __restrict__ A, int lda, const int* __restrict__ pairs, const int* __restrict__ npairs,
+                   int maxp, int c0, int c1, int c2, int c3)
+{
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+    const int q = blockIdx.x * (256 / kWave) + wv;
+    const int n1 = c1 - c0;
+    const int ncol = n1 + (c3 - c2);
+    const bool live = q < ncol;
+    const int col = !live ? 0 : (q < n1 ? c0 + q : c2 + (q - n1));
+    const int np = *npairs;
+    T* Ac = A + (size_t)col * lda;
+    T v[MAXPL];
+    int dst[MAXPL];
+#pragma unroll
+    for (int i = 0; i < MAXPL; ++i) {
+        const int e = ln + kWave * i;
+        dst[i] = -1;
+        v[i] = T(0);
+        if (live && e < np) {
+            const int s = pairs[e], d = pairs[maxp + e];
+            if (s >= 0 && s < m && d >= 0 && d < m) { v[i] = Ac[s]; dst[i] = d; }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXPL; ++i)
+        if (dst[i] >= 0) Ac[dst[i]] = v[i];
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of diagonal blocks. Block b (blockIdx.y) starts at row and column
+// k0 + b*bs and has order nb = min(bs, n - k0 - b*bs); its inverse goes to
+// X + b*bs*bs (column-major, ld bs). UPPER = false: unit lower triangle;
+// UPPER = true: upper triangle with its diagonal. One wave per column of the
+// inverse (forward or back substitution on a unit vector); the triangle the
+// workgroup needs is staged in LDS, packed by columns. bs <= 256.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline size_t trtri_lds_elems(int bs) { return (size_t)bs * (bs + 1) / 2; }
+
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(256)
+dense_trtri_kernel(int n, int k0, int bs, const T* __restrict__ A, int lda, T* __restrict__ X)
+{
+    extern __shared__ __align__(16) unsigned char tr_raw[];
+    T* S = reinterpret_cast<T*>(tr_raw);
+    const int b = blockIdx.y;
+    const int o = k0 + b * bs;
+    const int nb = min(bs, n - o);
+    const int j0 = blockIdx.x * 4;
+    if (nb <= 0 || j0 >= nb) return;
+    const T* D = A + o + (size_t)o * lda;
+    const int t = threadIdx.x, wv = t / kWave, ln = t % kWave;
+    // lower: columns p >= j0, rows p .. nb-1, column p at offset
+    //   sum_{q=j0}^{p-1} (nb - q);  upper: columns p < min(nb, j0+4), rows 0..p at p(p+1)/2
+    auto loff = [&](int p) -> int { return (p - j0) * nb - (p * (p - 1) / 2 - j0 * (j0 - 1) / 2); };
+    if (!UPPER) {
+        for (int p = j0 + wv; p < nb; p += 4) {
+            T* dstc = S + loff(p) - p;
+            for (int rr = p + ln; rr < nb; rr += kWave) dstc[rr] = D[rr + (size_t)p * lda];
+        }
+    } else {
+        const int pe = min(nb, j0 + 4);
+        for (int p = wv; p < pe; p += 4) {
+            T* dstc = S + p * (p + 1) / 2;
+            for (int rr = ln; rr <= p; rr += kWave) dstc[rr] = D[rr + (size_t)p * lda];
+        }
+    }
+    __syncthreads();
+    const int j = j0 + wv;
+    if (j >= nb) return;
+    T x[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) x[s] = (ln + kWave * s == j) ? T(1) : T(0);
+    if (!UPPER) {
+        for (int p = j; p < nb - 1; ++p) {
+            const int sp = p >> 6;
+            T xs = x[0];
+            if (sp == 1) xs = x[1];
+            else if (sp == 2) xs = x[2];
+            else if (sp == 3) xs = x[3];
+            const T xp = readlane_t(xs, p & 63);
+            const T* col = S + loff(p) - p;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int rr = ln + kWave * s;
+                if (rr > p && rr < nb) x[s] = fma(-col[rr], xp, x[s]);
+            }
+        }
+    } else {
+        for (int p = j; p >= 0; --p) {
+            const int sp = p >> 6;
+            T xs = x[0];
+            if (sp == 1) xs = x[1];
+            else if (sp == 2) xs = x[2];
+            else if (sp == 3) xs = x[3];
+            const T* col = S + p * (p + 1) / 2;
+            const T xp = readlane_t(xs, p & 63) / col[p];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int rr = ln + kWave * s;
+                if (rr == p) x[s] = xp;
+                else if (rr < p) x[s] = fma(-col[rr], xp, x[s]);
+            }
+        }
+    }
+    T* Xo = X + (size_t)b * bs * bs + (size_t)j * bs;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int rr = ln + kWave * s;
+        if (rr < nb) Xo[rr] = x[s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C := A B (SET) or C := C - A B, all column-major, 128 x 128 tiles per
+// workgroup (4 waves of 64 x 64 = 4 x 4 MFMA tiles). The product is formed
+// transposed (B^T as the first MFMA operand, A^T as the second), so each
+// accumulator register holds 16 consecutive rows of one column of C and the
+// stores are whole 64-byte (FP32) or 128-byte (FP64) column segments.
+// Workgroup ids are spread so that each XCD works on a contiguous run of
+// tiles (ids go round-robin over the 8 XCDs).
+//   MFMA operand layouts (lane l):  16x16x4 f64/f32: P[l%16][l/16], Q[l/16][l%16]
+//                                   16x16x32 bf16/f16: P[l%16][8(l/16)+v], Q[8(l/16)+v][l%16]
+//   accumulators: f64 D[4r + l/16][l%16], f32 D[4(l/16) + r][l%16]
+// ---------------------------------------------------------------------------
+constexpr int kGBM = 128, kGBN = 128;
+
+__device__ __forceinline__ int gemm_tile_id(int tiles)
+{
+    const int pid = blockIdx.x;
+    const int G = gridDim.x;
+    if (G >= 8 && (G & 7) == 0) {
+        const int per = G >> 3;
+        return (pid & 7) * per + (pid >> 3);
+    }
+    return pid;
+}
+
+template <typename T, int MODE, bool SET>
+__global__ void __launch_bounds__(256, 2)
+dense_gemm_kernel(int M, int N, int K, const T* __restrict__ A, int lda, const T* B, int ldb,
+                  T* C, int ldc, int tiles_m, int tiles_n)
+{
+    constexpr bool kWide = (MODE == kUpdBF16 || MODE == kUpdFP16 || MODE == kUpdBF16x3);
+    const int tid = gemm_tile_id(tiles_m * tiles_n);
+    if (tid >= tiles_m * tiles_n) return;
+    const int m0 = (tid % tiles_m) * kGBM;
+    const int n0 = (tid / tiles_m) * kGBN;
+    const int t = threadIdx.x;
+    const int w = t / kWave, l = t % kWave;
+    const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+    const int lr = l % 16, lq = l / 16;
+
+    if constexpr (!kWide) {
+        constexpr int BK = 16;
+        constexpr int LDP = kGBM + 4;
+        __shared__ T sA[2][BK][LDP];
+        __shared__ T sB[2][BK][LDP];
+        using acc_t = typename std::conditional<std::is_same<T, double>::value, d4_t, f4_t>::type;
+        acc_t acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+        T ra[8], rb[8];
+        const int ai = t & 127, ak = t >> 7;      // A tile: row ai, k = ak + 2q
+        const int bk = t & 15, bj = t >> 4;       // B tile: k = bk, column bj + 16q
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int kk = ak + 2 * q;
+                ra[q] = (m0 + ai < M && k0 + kk < K) ? A[(size_t)(m0 + ai) + (size_t)(k0 + kk) * lda] : T(0);
+                const int jj = bj + 16 * q;
+                rb[q] = (k0 + bk < K && n0 + jj < N) ? B[(size_t)(k0 + bk) + (size_t)(n0 + jj) * ldb] : T(0);
+            }
+        };
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sA[buf][ak + 2 * q][ai] = ra[q];
+                sB[buf][bk][bj + 16 * q] = rb[q];
+            }
+        };
+        const int nst = (K + BK - 1) / BK;
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int s = 0; s < nst; ++s) {
+            const int buf = s & 1;
+            if (s + 1 < nst) gload((s + 1) * BK);
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                T pf[4], qf[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    pf[x] = sB[buf][kk + lq][wc + 16 * x + lr];
+                    qf[x] = sA[buf][kk + lq][wr + 16 * x + lr];
+                }
+#pragma unroll
+                for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                    for (int ti = 0; ti < 4; ++ti) {
+                        if constexpr (std::is_same<T, double>::value)
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                        else
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                    }
+            }
+            if (s + 1 < nst) lstore(buf ^ 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) {
+                const int row = m0 + wr + 16 * ti + lr;
+                if (row >= M) continue;
+#pragma unroll
+                for (int rg = 0; rg < 4; ++rg) {
+                    const int cc = std::is_same<T, double>::value ? (4 * rg + lq) : (4 * lq + rg);
+                    const int col = n0 + wc + 16 * tj + cc;
+                    if (col < N) {
+                        T* cp = C + (size_t)row + (size_t)col * ldc;
+                        if constexpr (SET) *cp = acc[tj][ti][rg];
+                        else *cp = *cp - acc[tj][ti][rg];
+                    }
+                }
+            }
+    } else {
+        // FP32 storage, 16-bit MFMA inputs: tiles staged as 16-bit values
+        // with k contiguous (one 16-byte LDS read per operand fragment)
+        constexpr int BK = 64;
+        constexpr int LDK = BK + 8;
+        using h_t = typename std::conditional<MODE == kUpdFP16, _Float16, __bf16>::type;
+        using v8_t = typename std::conditional<MODE == kUpdFP16, h8_t, bf8_t>::type;
+        constexpr int kSplit = (MODE == kUpdBF16x3) ? 2 : 1;   // hi (and lo) parts
+        __shared__ __align__(16) h_t sA[kSplit][2][kGBM][LDK];
+        __shared__ __align__(16) h_t sB[kSplit][2][kGBN][LDK];
+        f4_t acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = f4_t{0.f, 0.f, 0.f, 0.f};
+        // A tile: thread owns row ai and k-groups ag + 2q (8 consecutive k each)
+        const int ai = t & 127, ag = t >> 7;
+        // B tile: 4-vectors e = t + 256 q: column e/16, k = 4 (e%16) .. +3
+        float ra[4][8], rb[8][4];
+        auto gload = [&](int k0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int kg = ag + 2 * q;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int kk = k0 + 8 * kg + v;
+                    ra[q][v] = (m0 + ai < M && kk < K) ? (float)A[(size_t)(m0 + ai) + (size_t)kk * lda] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = t + 256 * q;
+                const int jj = e >> 4, kv = e & 15;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int kk = k0 + 4 * kv + v;
+                    rb[q][v] = (n0 + jj < N && kk < K) ? (float)B[(size_t)kk + (size_t)(n0 + jj) * ldb] : 0.f;
+                }
+            }
+        };
+        typedef h_t v4h_t __attribute__((ext_vector_type(4)));
+        auto lstore = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int kg = ag + 2 * q;
+                v8_t hi, lo;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    hi[v] = (h_t)ra[q][v];
+                    lo[v] = (h_t)(ra[q][v] - (float)hi[v]);
+                }
+                *reinterpret_cast<v8_t*>(&sA[0][buf][ai][8 * kg]) = hi;
+                if constexpr (kSplit == 2) *reinterpret_cast<v8_t*>(&sA[kSplit - 1][buf][ai][8 * kg]) = lo;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = t + 256 * q;
+                const int jj = e >> 4, kv = e & 15;
+                v4h_t hi, lo;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    hi[v] = (h_t)rb[q][v];
+                    lo[v] = (h_t)(rb[q][v] - (float)hi[v]);
+                }
+                *reinterpret_cast<v4h_t*>(&sB[0][buf][jj][4 * kv]) = hi;
+                if constexpr (kSplit == 2) *reinterpret_cast<v4h_t*>(&sB[kSplit - 1][buf][jj][4 * kv]) = lo;
+            }
+        };
+        const int nst = (K + BK - 1) / BK;
+        gload(0);
+        lstore(0);
+        __syncthreads();
+        for (int s = 0; s < nst; ++s) {
+            const int buf = s & 1;
+            if (s + 1 < nst) gload((s + 1) * BK);
+#pragma unroll
+            for (int kb = 0; kb < BK; kb += 32) {
+                v8_t pf[4], qf[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    pf[x] = *reinterpret_cast<const v8_t*>(&sB[0][buf][wc + 16 * x + lr][kb + 8 * lq]);
+                    qf[x] = *reinterpret_cast<const v8_t*>(&sA[0][buf][wr + 16 * x + lr][kb + 8 * lq]);
+                }
+                if constexpr (MODE == kUpdBF16x3) {
+                    v8_t pl[4], ql[4];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) {
+                        pl[x] = *reinterpret_cast<const v8_t*>(&sB[1][buf][wc + 16 * x + lr][kb + 8 * lq]);
+                        ql[x] = *reinterpret_cast<const v8_t*>(&sA[1][buf][wr + 16 * x + lr][kb + 8 * lq]);
+                    }
+#pragma unroll
+                    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                        for (int ti = 0; ti < 4; ++ti) {
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pl[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pf[tj], ql[ti], acc[tj][ti], 0, 0, 0);
+                        }
+                }
+#pragma unroll
+                for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                    for (int ti = 0; ti < 4; ++ti) {
+                        if constexpr (MODE == kUpdFP16)
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                        else
+                            acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                    }
+            }
+            if (s + 1 < nst) lstore(buf ^ 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti) {
+                const int row = m0 + wr + 16 * ti + lr;
+                if (row >= M) continue;
+#pragma unroll
+                for (int rg = 0; rg < 4; ++rg) {
+                    const int col = n0 + wc + 16 * tj + 4 * lq + rg;
+                    if (col < N) {
+                        T* cp = C + (size_t)row + (size_t)col * ldc;
+                        if constexpr (SET) *cp = acc[tj][ti][rg];
+                        else *cp = *cp - acc[tj][ti][rg];
+                    }
+                }
+            }
+    }
+}
+
+template <typename T>
+__global__ void dense_copy_kernel(int rows, int cols, const T* __restrict__ S, int lds, T* __restrict__ D, int ldd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < rows && j < cols) D[(size_t)i + (size_t)j * ldd] = S[(size_t)i + (size_t)j * lds];
+}
+
+__global__ void dense_fill_int_kernel(int* p, int v, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+int device_cu_count()
+{
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 1;
+    return v;
+}
+
+int dense_panel_gmax(int m) { return (int)ceildiv(std::max(m, 1), kDensePanelRows); }
+
+static size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t dense_work_bytes(int n, int elem, int gmax)
+{
+    const int PW = elem == 8 ? DenseShape<double>::PW : DenseShape<float>::PW;
+    const int NB = elem == 8 ? DenseShape<double>::NB : DenseShape<float>::NB;
+    size_t b = 0;
+    b += a256(sizeof(int) * 2 * gmax);                  // tag
+    b += a256(sizeof(int) * 2 * gmax);                  // cpos
+    b += a256((size_t)elem * 2 * gmax * PW);            // crow
+    b += a256(sizeof(int) * 4);                         // err
+    b += a256(sizeof(int) * 2 * 2 * NB);                // pairs
+    b += a256(sizeof(int) * 4);                         // npairs
+    b += a256((size_t)elem * NB * NB);                  // X
+    if (elem == 4) b += a256((size_t)elem * NB * std::max(n, 1));   // W (FP32 only)
+    return b;
+}
+
+void dense_work_carve(void* base, int n, int elem, int gmax, DenseWork& w)
+{
+    const int PW = elem == 8 ? DenseShape<double>::PW : DenseShape<float>::PW;
+    const int NB = elem == 8 ? DenseShape<double>::NB : DenseShape<float>::NB;
+    char* p = static_cast<char*>(base);
+    w.tag = reinterpret_cast<int*>(p);   p += a256(sizeof(int) * 2 * gmax);
+    w.cpos = reinterpret_cast<int*>(p);  p += a256(sizeof(int) * 2 * gmax);
+    w.crow = p;                          p += a256((size_t)elem * 2 * gmax * PW);
+    w.err = reinterpret_cast<int*>(p);   p += a256(sizeof(int) * 4);
+    w.pairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 2 * 2 * NB);
+    w.npairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 4);
+    w.X = p;                             p += a256((size_t)elem * NB * NB);
+    w.W = elem == 4 ? p : nullptr;
+    w.gmax = gmax;
+}
+
+template <typename T>
+magma_int_t dense_trtri_launch(bool upper, int n, int k0, int bs, int nblk, const T* A, int lda, T* X, hipStream_t st)
+{
+    const size_t lds = sizeof(T) * trtri_lds_elems(bs);
+    dim3 grid((unsigned)ceildiv(bs, 4), (unsigned)nblk);
+    if (upper) {
+        MGI_HIP_CHECK(hipFuncSetAttribute((const void*)dense_trtri_kernel<T, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        dense_trtri_kernel<T, true><<<grid, 256, lds, st>>>(n, k0, bs, A, lda, X);
+    } else {
+        MGI_HIP_CHECK(hipFuncSetAttribute((const void*)dense_trtri_kernel<T, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        dense_trtri_kernel<T, false><<<grid, 256, lds, st>>>(n, k0, bs, A, lda, X);
+    }
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+template magma_int_t dense_trtri_launch<double>(bool, int, int, int, int, const double*, int, double*, hipStream_t);
+template magma_int_t dense_trtri_launch<float>(bool, int, int, int, int, const float*, int, float*, hipStream_t);
+
+namespace {
+
+template <typename T, int MODE, bool SET>
+magma_int_t gemm_launch(int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+                        hipStream_t st)
+{
+    if (M <= 0 || N <= 0) return MAGMA_SUCCESS;
+    const int tm = (int)ceildiv(M, kGBM), tn = (int)ceildiv(N, kGBN);
+    const int total = tm * tn;
+    const int grid = total >= 8 ? (int)roundup(total, 8) : total;
+    dense_gemm_kernel<T, MODE, SET><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, tm, tn);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+template <typename T>
+magma_int_t update_launch(int mode, int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+                          hipStream_t st)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        return gemm_launch<double, kUpdF64, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+    } else {
+        switch (mode) {
+            case kUpdF32:    return gemm_launch<float, kUpdF32, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+            case kUpdBF16:   return gemm_launch<float, kUpdBF16, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+            case kUpdFP16:   return gemm_launch<float, kUpdFP16, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+            case kUpdBF16x3: return gemm_launch<float, kUpdBF16x3, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+            default:         return MAGMA_ERR_NOT_SUPPORTED;
+        }
+    }
+}
+
+template <typename T>
+magma_int_t panel_launch(int m, int k, int jb, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
+                         hipStream_t st)
+{
+    constexpr int PW = DenseShape<T>::PW;
+    const int G = (int)ceildiv(m - k, kDensePanelRows);
+    if (G > w.gmax) return MAGMA_ERR_NOT_SUPPORTED;
+    dense_panel_kernel<T, PW><<<G, kDensePanelRows, 0, st>>>(m, k, jb, A, lda, ipiv, info, w);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+// Interchanges ipiv[s .. s+cnt) applied to the columns [c0, c1) and [c2, c3).
+template <typename T>
+magma_int_t swaps_launch(int m, const int* ipiv, int s, int cnt, T* A, int lda, const DenseWork& w, int c0, int c1,
+                         int c2, int c3, hipStream_t st)
+{
+    constexpr int NB = DenseShape<T>::NB;
+    const int ncol = (c1 - c0) + (c3 - c2);
+    if (ncol <= 0 || cnt <= 0) return MAGMA_SUCCESS;
+    dense_pivmap_kernel<<<1, 512, 0, st>>>(ipiv, s, cnt, w.pairs, w.npairs, 2 * NB);
+    MGI_LAUNCH_CHECK();
+    dense_laswp_kernel<T, 2 * NB / kWave><<<(unsigned)ceildiv(ncol, 4), 256, 0, st>>>(
+        m, A, lda, w.pairs, w.npairs, 2 * NB, c0, c1, c2, c3);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+// U12 := L11^{-1} A[k:k+jb, cb:ce) (jb x jb unit lower L11 at (k,k)), then
+// A[k+jb:m, cb:ce) -= A[k+jb:m, k:k+jb) U12. FP64 (jb <= 128 = one GEMM row
+// tile): each workgroup of the U12 product owns all jb rows of its columns
+// and writes them only after reading them, so the product runs in place.
+// FP32 (jb up to 256, two row tiles): U12 goes through W, then into A.
+template <typename T>
+magma_int_t trsm_update(int mode, int m, int k, int jb, int cb, int ce, T* A, int lda, const DenseWork& w,
+                        hipStream_t st)
+{
+    constexpr int NB = DenseShape<T>::NB;
+    const int nc = ce - cb;
+    if (nc <= 0 || jb <= 0) return MAGMA_SUCCESS;
+    T* X = reinterpret_cast<T*>(w.X);
+    magma_int_t rc = dense_trtri_launch<T>(false, k + jb, k, NB, 1, A, lda, X, st);
+    if (rc != MAGMA_SUCCESS) return rc;
+    T* A12 = A + k + (size_t)cb * lda;
+    const T* U12 = A12;
+    int ldu = lda;
+    if constexpr (std::is_same<T, double>::value) {
+        static_assert(DenseShape<double>::NB <= kGBM, "in-place U12 needs one row tile");
+        rc = gemm_launch<double, kUpdF64, true>(jb, nc, jb, X, NB, A12, lda, A12, lda, st);
+        if (rc != MAGMA_SUCCESS) return rc;
+    } else {
+        T* W = reinterpret_cast<T*>(w.W);
+        rc = gemm_launch<float, kUpdF32, true>(jb, nc, jb, X, NB, A12, lda, W, NB, st);
+        if (rc != MAGMA_SUCCESS) return rc;
+        dense_copy_kernel<T><<<dim3((unsigned)ceildiv(jb, 256), (unsigned)nc), 256, 0, st>>>(jb, nc, W, NB, A12,
+                                                                                               lda);
+        MGI_LAUNCH_CHECK();
+        U12 = W;
+        ldu = NB;
+    }
+    if (m - k - jb > 0)
+        rc = update_launch<T>(mode, m - k - jb, nc, jb, A + (k + jb) + (size_t)k * lda, lda, U12, ldu,
+                              A + (k + jb) + (size_t)cb * lda, lda, st);
+    return rc;
+}
+
+template <typename T>
+magma_int_t dense_getrf_t(int mode, int m, int n, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
+                          hipStream_t st)
+{
+    constexpr int PW = DenseShape<T>::PW;
+    constexpr int NB = DenseShape<T>::NB;
+    if (dense_panel_gmax(m) > w.gmax) return MAGMA_ERR_NOT_SUPPORTED;
+    dense_fill_int_kernel<<<(unsigned)ceildiv(2 * w.gmax, 256), 256, 0, st>>>(w.tag, -1, 2 * w.gmax);
+    dense_fill_int_kernel<<<(unsigned)ceildiv(2 * w.gmax, 256), 256, 0, st>>>(w.cpos, -1, 2 * w.gmax);
+    dense_fill_int_kernel<<<1, 32, 0, st>>>(w.err, 0, 1);
+    dense_fill_int_kernel<<<1, 32, 0, st>>>(info, 0, 1);
+    MGI_LAUNCH_CHECK();
+    const int minmn = std::min(m, n);
+    magma_int_t rc = MAGMA_SUCCESS;
+    for (int k = 0; k < minmn; k += NB) {
+        const int jb = std::min(NB, minmn - k);
+        const int jb1 = std::min(PW, jb);
+        if ((rc = panel_launch<T>(m, k, jb1, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+        if (jb > jb1) {
+            const int jb2 = jb - jb1;
+            if ((rc = swaps_launch<T>(m, ipiv, k, jb1, A, lda, w, k + jb1, k + jb, 0, 0, st)) != MAGMA_SUCCESS)
+                return rc;
+            if ((rc = trsm_update<T>(mode, m, k, jb1, k + jb1, k + jb, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = panel_launch<T>(m, k + jb1, jb2, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = swaps_launch<T>(m, ipiv, k + jb1, jb2, A, lda, w, k, k + jb1, 0, 0, st)) != MAGMA_SUCCESS)
+                return rc;
+        }
+        if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w, 0, k, k + jb, n, st)) != MAGMA_SUCCESS) return rc;
+        if ((rc = trsm_update<T>(mode, m, k, jb, k + jb, n, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+    }
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace
+
+magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
+                        hipStream_t stream)
+{
+    (void)mode;
+    return dense_getrf_t<double>(kUpdF64, m, n, dA, ldda, d_ipiv, d_info, w, stream);
+}
+
+magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
+                         hipStream_t stream)
+{
+    if (mode != kUpdF32 && mode != kUpdBF16 && mode != kUpdFP16 && mode != kUpdBF16x3)
+        return MAGMA_ERR_NOT_SUPPORTED;
+    return dense_getrf_t<float>(mode, m, n, dA, ldda, d_ipiv, d_info, w, stream);
+}
+
+magma_int_t dense_check_err(const DenseWork& w, hipStream_t stream)
+{
+    int h = 0;
+    MGI_HIP_CHECK(hipMemcpyAsync(&h, w.err, sizeof(int), hipMemcpyDeviceToHost, stream));
+    MGI_HIP_CHECK(hipStreamSynchronize(stream));
+    if (h != 0) {
+        std::fprintf(stderr, "magma_mi355x: dense LU panel wait timed out\n");
+        return MAGMA_ERR_UNKNOWN;
+    }
+    return MAGMA_SUCCESS;
+}
+
+}  // namespace mgi

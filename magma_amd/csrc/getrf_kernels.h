@@ -24,7 +24,8 @@ struct SizeArgs {
     const int* n_arr;
     const int* ld_arr;
     const int* map;          // optional: workgroup index -> matrix index
-    __device__ __forceinline__ int mat(int bi) const { return map ? map[bi] : bi; }
+    int base = 0;            // first workgroup index of this launch (a batch split over launches)
+    __device__ __forceinline__ int mat(int bi) const { return map ? map[base + bi] : base + bi; }
     // each per-matrix array is optional on its own; a null one means the
     // uniform value
     __device__ __forceinline__ void get(int b, int& mm, int& nn, int& ll) const

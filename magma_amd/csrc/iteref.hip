@@ -21,6 +21,7 @@
 #include "magma_internal.h"
 #include "lu_device.h"
 #include "getrf_kernels.h"
+#include "dense_kernels.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -30,7 +31,9 @@
 namespace mgi {
 
 constexpr int kIrThreads = 256;
-constexpr int kGmresMaxRestart = 40;
+// single systems above this order go to the dense LU on the whole device
+constexpr int kDenseIrMinN = 512;
+constexpr int kGmresMaxRestart = 50;   // restrt = inner_itermax, src/dxgesv_gmres_gpu.cpp:212
 constexpr int kGmresCheck = 4;       // inner steps between host checks of the open cycles
 
 struct IrState {
@@ -736,6 +739,8 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs, double*
     }
     if (n == 0 || nrhs == 0 || batchCount == 0) return 0;
     if (queue == nullptr) return MAGMA_ERR_NOT_INITIALIZED;
+    // the refinement kernels put the batch in grid y
+    if (batchCount > kMaxGridYZ) return MAGMA_ERR_NOT_SUPPORTED;
     if (!getrf_blocked_supported(n)) return MAGMA_ERR_NOT_SUPPORTED;
     hipSetDevice(queue->device);
     return iteref_batched(0, n, nrhs, dA_array, ldda, dB_array, lddb, dX_array, lddx, mode,
@@ -768,12 +773,30 @@ magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int
         return *info;
     }
     if (n == 0 || nrhs == 0) return 0;
-    if (!getrf_blocked_supported(n)) return MAGMA_ERR_NOT_SUPPORTED;
+    if (n > kDenseIrMinN || !getrf_blocked_supported(n)) {
+        // one large system: the dense LU on the whole device
+        const double t0 = magma_wtime();
+        int it = 0, inf = 0;
+        const magma_int_t rc = dense_iteref(trans == MagmaNoTrans ? 0 : 1, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb,
+                                            dX, lddx, dworkd, dworks, mode, solver_type == Magma_REFINE_GMSTRS, &it,
+                                            &inf);
+        if (facto_time) *facto_time = magma_wtime() - t0;
+        if (rc != MAGMA_SUCCESS) {
+            *info = rc;
+            return rc;
+        }
+        *iter = it;
+        *info = inf;
+        return *info;
+    }
     magma_queue_t q = nullptr;
     int dev = 0;
     hipGetDevice(&dev);
     magma_queue_create_internal(dev, &q, __func__, __FILE__, __LINE__);
-    if (q == nullptr) return MAGMA_ERR_UNKNOWN;
+    if (q == nullptr) {
+        *info = MAGMA_ERR_UNKNOWN;
+        return MAGMA_ERR_UNKNOWN;
+    }
     const double t0 = magma_wtime();
     double** ptrs = nullptr;
     int* small = nullptr;
@@ -809,7 +832,11 @@ magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int
     hipFree(ptrs);
     hipFree(small);
     magma_queue_destroy_internal(q, __func__, __FILE__, __LINE__);
-    return rc != MAGMA_SUCCESS ? rc : *info;
+    if (rc != MAGMA_SUCCESS) {
+        *info = rc;
+        return rc;
+    }
+    return *info;
 }
 
 // include/magma_htc.h:27: FP16-input LU + FGMRES (src/dxgesv_gmres_gpu.cpp:1143-1152).
@@ -819,9 +846,8 @@ magma_int_t magma_dhgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
                                     magmaFloat_ptr dworks, magma_int_t* iter, magma_int_t* info)
 {
     real_Double_t ft;
-    magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
-                           Magma_PREC_SHT, Magma_REFINE_GMSTRS, iter, info, &ft);
-    return *info;
+    return magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
+                                  Magma_PREC_SHT, Magma_REFINE_GMSTRS, iter, info, &ft);
 }
 
 magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs, magmaDouble_ptr dA,
@@ -830,9 +856,8 @@ magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
                                     magmaFloat_ptr dworks, magma_int_t* iter, magma_int_t* info)
 {
     real_Double_t ft;
-    magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
-                           Magma_PREC_SBT, Magma_REFINE_GMSTRS, iter, info, &ft);
-    return *info;
+    return magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
+                                  Magma_PREC_SBT, Magma_REFINE_GMSTRS, iter, info, &ft);
 }
 
 }  // extern "C"

@@ -33,8 +33,9 @@ void* queue_scratch(magma_queue_t queue, size_t bytes);
 // style codes to the caller.
 void hip_report(hipError_t err, const char* what, const char* file, int line);
 
-// Largest batch a single grid dimension handles; drivers split larger batches.
-constexpr magma_int_t kMaxGridBatch = 65535 * 16;
+// Largest grid extent in y and z: launches that put the batch there split it
+// into launches of at most this many matrices.
+constexpr int kMaxGridYZ = 65535;
 
 }  // namespace mgi
 

@@ -96,6 +96,14 @@ def rewrite_launches(s: str) -> str:
 
 
 DYN = re.compile(r"extern\s+__shared__\s+(?:__align__\(\d+\)\s+)?unsigned\s+char\s+(\w+)\[\];")
+# static LDS declarations (per workgroup storage in the emulation)
+SHV = re.compile(r"__shared__\s+(?:__align__\(\d+\)\s+)?([\w:<>, ]+?)\s+(\w+)((?:\[[^\];]+\])*)\s*;")
+_SHV_ID = [0]
+
+
+def _shvar(m: re.Match) -> str:
+    _SHV_ID[0] += 1
+    return f"auto& {m.group(2)} = emu::shvar<{m.group(1)}{m.group(3)}>({_SHV_ID[0]});"
 
 
 def transform(src: Path, dst: Path) -> list[Path]:
@@ -104,10 +112,12 @@ def transform(src: Path, dst: Path) -> list[Path]:
     for f in sorted(src.glob("*.hip")) + sorted(src.glob("*.h")):
         s = f.read_text()
         s = DYN.sub(r"unsigned char* \1 = emu::dyn_shared();", s)
+        s = SHV.sub(_shvar, s)
         s = rewrite_launches(s)
         s = s.replace('"../../include/magma_mi355x.h"', "<magma_mi355x.h>")
         out = dst / f.name.replace(".hip", ".cpp")
-        out.write_text(s)
+        if not out.exists() or out.read_text() != s:
+            out.write_text(s)
         if out.suffix == ".cpp":
             cpps.append(out)
     return cpps
@@ -125,8 +135,13 @@ def build(src: Path | None = None) -> Path:
     flags = ["-w", "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-fPIC",
              f"-I{HERE / 'include'}", f"-I{ROOT / 'include'}"]
 
+    headers = [h for h in gen.glob("*.h")] + [HERE / "include" / "hip" / "hip_runtime.h"]
+    newest_h = max(h.stat().st_mtime for h in headers)
+
     def compile_one(c: Path) -> Path:
         o = objd / (c.stem + ".o")
+        if o.exists() and o.stat().st_mtime > max(c.stat().st_mtime, newest_h):
+            return o
         r = subprocess.run([cxx, *flags, "-c", str(c), "-o", str(o)], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"compile of {c.name} failed:\n{r.stderr[-4000:]}")

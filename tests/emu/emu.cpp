@@ -1,8 +1,14 @@
-// Fiber executor for the HIP emulation header: the lanes of one workgroup are
-// ucontext fibers on the calling thread, switched at barriers and wave
-// exchanges; workgroups run one after another.
+// Fiber executor for the HIP emulation header: the lanes of a workgroup are
+// fibers on the calling thread, switched at barriers, wave exchanges and
+// waits (s_sleep). Up to EMU_WG_WINDOW workgroups of a grid (default 1) run
+// interleaved, round robin over their runnable lanes, so kernels whose
+// workgroups wait for one another can run, and orderings that in-order
+// execution would hide can show up.
 #include <hip/hip_runtime.h>
 #include <sanitizer/common_interface_defs.h>
+
+#include <chrono>
+#include <map>
 
 // Minimal x86-64 context switch: saves the callee-saved registers on the
 // current stack, stores rsp, loads the other stack and returns into it.
@@ -41,23 +47,32 @@ constexpr int kMaxThreads = 1024;
 constexpr size_t kStack = 512 * 1024;
 
 enum State { kRun, kWaitBlock, kWaitWave, kDone };
+struct Ctx;
 struct Fiber {
     void* rsp = nullptr;
     char* stack = nullptr;
     State st = kRun;
     EmuIdx tid;
+    int local = 0;       // thread index within the workgroup
+    Ctx* ctx = nullptr;
+};
+
+struct Ctx {                       // one workgroup in flight
+    Block blk;
+    EmuIdx bidx;
+    int n = 0, live = 0, block_arrived = 0;
+    std::vector<int> wave_live, wave_arrived;
+    std::vector<Fiber*> f;
+    unsigned char* heap = nullptr;
+    int done = 0;
 };
 
 struct Sched {
-    std::vector<Fiber> f;
-    int n = 0;
-    int cur = -1;
+    std::vector<Fiber> pool;
+    Fiber* cur = nullptr;
     void* main_rsp = nullptr;
     const void* main_bottom = nullptr;
     size_t main_size = 0;
-    int live = 0;
-    int block_arrived = 0;
-    std::vector<int> wave_live, wave_arrived;
     const std::function<void()>* fn = nullptr;
 };
 Sched S;
@@ -66,24 +81,24 @@ void to_main()
 {
     void* fake = nullptr;
     __sanitizer_start_switch_fiber(&fake, S.main_bottom, S.main_size);
-    emu_switch(&S.f[S.cur].rsp, S.main_rsp);
+    emu_switch(&S.cur->rsp, S.main_rsp);
     __sanitizer_finish_switch_fiber(fake, &S.main_bottom, &S.main_size);
 }
 
-void release_block_if_complete()
+void release_block_if_complete(Ctx& c)
 {
-    if (S.block_arrived > 0 && S.block_arrived == S.live) {
-        for (int i = 0; i < S.n; ++i)
-            if (S.f[i].st == kWaitBlock) S.f[i].st = kRun;
-        S.block_arrived = 0;
+    if (c.block_arrived > 0 && c.block_arrived == c.live) {
+        for (Fiber* f : c.f)
+            if (f->st == kWaitBlock) f->st = kRun;
+        c.block_arrived = 0;
     }
 }
-void release_wave_if_complete(int w)
+void release_wave_if_complete(Ctx& c, int w)
 {
-    if (S.wave_arrived[w] > 0 && S.wave_arrived[w] == S.wave_live[w]) {
-        for (int i = 64 * w; i < std::min(S.n, 64 * w + 64); ++i)
-            if (S.f[i].st == kWaitWave) S.f[i].st = kRun;
-        S.wave_arrived[w] = 0;
+    if (c.wave_arrived[w] > 0 && c.wave_arrived[w] == c.wave_live[w]) {
+        for (int i = 64 * w; i < std::min(c.n, 64 * w + 64); ++i)
+            if (c.f[i]->st == kWaitWave) c.f[i]->st = kRun;
+        c.wave_arrived[w] = 0;
     }
 }
 
@@ -91,42 +106,118 @@ void entry()
 {
     __sanitizer_finish_switch_fiber(nullptr, &S.main_bottom, &S.main_size);
     (*S.fn)();
-    const int me = S.cur;
-    const int w = me / 64;
-    cur_block->waves[w].alive[me % 64] = false;
-    S.f[me].st = kDone;
-    --S.live;
-    --S.wave_live[w];
-    release_block_if_complete();
-    release_wave_if_complete(w);
+    Fiber* me = S.cur;
+    Ctx& c = *me->ctx;
+    const int w = me->local / 64;
+    c.blk.waves[w].alive[me->local % 64] = false;
+    me->st = kDone;
+    --c.live;
+    --c.wave_live[w];
+    release_block_if_complete(c);
+    release_wave_if_complete(c, w);
     __sanitizer_start_switch_fiber(nullptr, S.main_bottom, S.main_size);
     void* dead;
     emu_switch(&dead, S.main_rsp);
     __builtin_unreachable();
 }
+
+int window_size()
+{
+    const char* e = std::getenv("EMU_WG_WINDOW");
+    const int v = e ? std::atoi(e) : 1;
+    return v < 1 ? 1 : v;
+}
+
+void start_block(Ctx& c, unsigned bx, unsigned by, unsigned bz, dim3 block, size_t shm, Fiber* fibers)
+{
+    const int n = block.x * block.y * block.z;
+    const int nw = (n + 63) / 64;
+    c.blk.waves = std::vector<Wave>(nw);
+    c.blk.shvars.clear();
+    c.wave_live.assign(nw, 0);
+    c.wave_arrived.assign(nw, 0);
+    for (int w = 0; w < nw; ++w) {
+        const int cnt = std::min(64, n - 64 * w);
+        c.wave_live[w] = cnt;
+        for (int l = 0; l < 64; ++l) c.blk.waves[w].alive[l] = l < cnt;
+    }
+    // uninitialised LDS holds garbage on the GPU
+    c.heap = shm ? new unsigned char[shm] : nullptr;
+    if (c.heap) std::memset(c.heap, 0x7f, shm);
+    c.blk.dyn = c.heap;
+    c.bidx.x = bx; c.bidx.y = by; c.bidx.z = bz;
+    c.n = n;
+    c.live = n;
+    c.block_arrived = 0;
+    c.done = 0;
+    c.f.assign(n, nullptr);
+    for (int t = 0; t < n; ++t) {
+        Fiber& f = fibers[t];
+        c.f[t] = &f;
+        f.ctx = &c;
+        f.local = t;
+        f.st = kRun;
+        f.tid.x = t % block.x;
+        f.tid.y = (t / block.x) % block.y;
+        f.tid.z = t / (block.x * block.y);
+        // initial frame: six saved registers, then the return address;
+        // entry() starts with rsp % 16 == 8 as after a call
+        uintptr_t top = ((uintptr_t)(f.stack + kStack)) & ~(uintptr_t)15;
+        void** sp = (void**)(top - 8);
+        *--sp = (void*)&entry;
+        for (int r = 0; r < 6; ++r) *--sp = nullptr;
+        f.rsp = sp;
+    }
+}
 }  // namespace
 
 void block_sync()
 {
-    ++S.block_arrived;
-    if (S.block_arrived == S.live) {
-        release_block_if_complete();
+    Ctx& c = *S.cur->ctx;
+    ++c.block_arrived;
+    if (c.block_arrived == c.live) {
+        release_block_if_complete(c);
         return;
     }
-    S.f[S.cur].st = kWaitBlock;
+    S.cur->st = kWaitBlock;
     to_main();
 }
 
 void wave_sync()
 {
-    const int w = S.cur / 64;
-    ++S.wave_arrived[w];
-    if (S.wave_arrived[w] == S.wave_live[w]) {
-        release_wave_if_complete(w);
+    Ctx& c = *S.cur->ctx;
+    const int w = S.cur->local / 64;
+    ++c.wave_arrived[w];
+    if (c.wave_arrived[w] == c.wave_live[w]) {
+        release_wave_if_complete(c, w);
         return;
     }
-    S.f[S.cur].st = kWaitWave;
+    S.cur->st = kWaitWave;
     to_main();
+}
+
+void yield()
+{
+    if (S.cur) to_main();
+}
+
+unsigned long long realtime()
+{
+    using clk = std::chrono::steady_clock;
+    return (unsigned long long)(std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                    clk::now().time_since_epoch()).count() / 10);
+}
+
+void* shared_var(int id, size_t bytes)
+{
+    auto& m = cur_block->shvars;
+    auto it = m.find(id);
+    if (it == m.end()) {
+        std::vector<unsigned char> v(bytes + 16, 0x7f);
+        it = m.emplace(id, std::move(v)).first;
+    }
+    uintptr_t p = (uintptr_t)it->second.data();
+    return (void*)((p + 15) & ~(uintptr_t)15);
 }
 
 void launch_impl(dim3 grid, dim3 block, size_t shm, const std::function<void()>& fn)
@@ -137,78 +228,70 @@ void launch_impl(dim3 grid, dim3 block, size_t shm, const std::function<void()>&
         std::abort();
     }
     if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
-    if ((int)S.f.size() < n) {
-        const int old = (int)S.f.size();
-        S.f.resize(n);
-        for (int i = old; i < n; ++i) S.f[i].stack = (char*)std::malloc(kStack);
+    const long total = (long)grid.x * grid.y * grid.z;
+    const int win = (int)std::min<long>(window_size(), total);
+    const size_t need = (size_t)n * win;
+    if (S.pool.size() < need) {
+        const size_t old = S.pool.size();
+        // fibers hold pointers into the pool: grow it only while idle
+        S.pool.resize(need);
+        for (size_t i = old; i < need; ++i) S.pool[i].stack = (char*)std::malloc(kStack);
     }
-    for (unsigned bz = 0; bz < grid.z; ++bz)
-        for (unsigned by = 0; by < grid.y; ++by)
-            for (unsigned bx = 0; bx < grid.x; ++bx) {
-                Block blk;
-                const int nw = (n + 63) / 64;
-                blk.waves = std::vector<Wave>(nw);
-                S.wave_live.assign(nw, 0);
-                S.wave_arrived.assign(nw, 0);
-                for (int w = 0; w < nw; ++w) {
-                    const int cnt = std::min(64, n - 64 * w);
-                    S.wave_live[w] = cnt;
-                    for (int l = 0; l < 64; ++l) blk.waves[w].alive[l] = l < cnt;
-                }
-                // uninitialised LDS holds garbage on the GPU
-                unsigned char* heap = shm ? new unsigned char[shm] : nullptr;
-                if (heap) std::memset(heap, 0x7f, shm);
-                blk.dyn = heap;
-                cur_block = &blk;
-                blockIdx.x = bx; blockIdx.y = by; blockIdx.z = bz;
-                blockDim.x = block.x; blockDim.y = block.y; blockDim.z = block.z;
-                gridDim.x = grid.x; gridDim.y = grid.y; gridDim.z = grid.z;
-                S.n = n;
-                S.live = n;
-                S.block_arrived = 0;
-                S.fn = &fn;
-                for (int t = 0; t < n; ++t) {
-                    Fiber& f = S.f[t];
-                    f.st = kRun;
-                    f.tid.x = t % block.x;
-                    f.tid.y = (t / block.x) % block.y;
-                    f.tid.z = t / (block.x * block.y);
-                    // initial frame: six saved registers, then the return
-                    // address; entry() starts with rsp % 16 == 8 as after a call
-                    uintptr_t top = ((uintptr_t)(f.stack + kStack)) & ~(uintptr_t)15;
-                    void** sp = (void**)(top - 8);          // slot above the return address (alignment pad)
-                    *--sp = (void*)&entry;                  // return address
-                    for (int r = 0; r < 6; ++r) *--sp = nullptr;
-                    f.rsp = sp;
-                }
-                // round-robin over runnable fibers until all are done
-                int done = 0;
-                while (done < n) {
-                    bool progressed = false;
-                    done = 0;
-                    for (int t = 0; t < n; ++t) {
-                        Fiber& f = S.f[t];
-                        if (f.st == kDone) { ++done; continue; }
-                        if (f.st != kRun) continue;
-                        S.cur = t;
-                        cur_lane_global = t;
-                        threadIdx = f.tid;
-                        void* fake = nullptr;
-                        __sanitizer_start_switch_fiber(&fake, f.stack, kStack);
-                        emu_switch(&S.main_rsp, f.rsp);
-                        __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
-                        progressed = true;
-                        if (f.st == kDone) ++done;
-                    }
-                    if (!progressed && done < n) {
-                        std::fprintf(stderr, "emu: deadlock in block (%u,%u,%u): live %d, block arrived %d\n",
-                                     bx, by, bz, S.live, S.block_arrived);
-                        std::abort();
-                    }
-                }
-                delete[] heap;
-                cur_block = nullptr;
+    S.fn = &fn;
+    blockDim.x = block.x; blockDim.y = block.y; blockDim.z = block.z;
+    gridDim.x = grid.x; gridDim.y = grid.y; gridDim.z = grid.z;
+    std::vector<Ctx> ctx(win);
+    std::vector<bool> busy(win, false);
+    long next = 0, finished = 0;
+    auto block_coords = [&](long id, unsigned& bx, unsigned& by, unsigned& bz) {
+        bx = (unsigned)(id % grid.x);
+        by = (unsigned)((id / grid.x) % grid.y);
+        bz = (unsigned)(id / ((long)grid.x * grid.y));
+    };
+    while (finished < total) {
+        // fill free slots with the next workgroups, in order
+        for (int s = 0; s < win && next < total; ++s) {
+            if (busy[s]) continue;
+            unsigned bx, by, bz;
+            block_coords(next++, bx, by, bz);
+            start_block(ctx[s], bx, by, bz, block, shm, &S.pool[(size_t)s * n]);
+            busy[s] = true;
+        }
+        bool progressed = false;
+        for (int s = 0; s < win; ++s) {
+            if (!busy[s]) continue;
+            Ctx& c = ctx[s];
+            for (int t = 0; t < n; ++t) {
+                Fiber& f = *c.f[t];
+                if (f.st != kRun) continue;
+                S.cur = &f;
+                cur_block = &c.blk;
+                cur_lane_global = t;
+                threadIdx = f.tid;
+                blockIdx = c.bidx;
+                void* fake = nullptr;
+                __sanitizer_start_switch_fiber(&fake, f.stack, kStack);
+                emu_switch(&S.main_rsp, f.rsp);
+                __sanitizer_finish_switch_fiber(fake, nullptr, nullptr);
+                progressed = true;
             }
+            int done = 0;
+            for (int t = 0; t < n; ++t) done += c.f[t]->st == kDone;
+            if (done == n) {
+                delete[] c.heap;
+                c.heap = nullptr;
+                c.blk.shvars.clear();
+                busy[s] = false;
+                ++finished;
+            }
+        }
+        if (!progressed && finished < total) {
+            std::fprintf(stderr, "emu: deadlock: no runnable lane in the window\n");
+            std::abort();
+        }
+    }
+    S.cur = nullptr;
+    cur_block = nullptr;
 }
 }  // namespace emu
 

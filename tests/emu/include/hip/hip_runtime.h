@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -57,11 +58,19 @@ struct Wave {
 struct Block {
     std::vector<Wave> waves;
     unsigned char* dyn = nullptr;
+    std::map<int, std::vector<unsigned char>> shvars;   // static LDS of this workgroup
 };
 extern Block* cur_block;
 extern int cur_lane_global;
 void block_sync();
 void wave_sync();
+void yield();
+unsigned long long realtime();
+void* shared_var(int id, size_t bytes);
+// a __shared__ declaration: storage of this workgroup (the build rewrites
+// `__shared__ T name[..];` into `auto& name = emu::shvar<T[..]>(id);`)
+template <class A>
+inline A& shvar(int id) { return *reinterpret_cast<A*>(shared_var(id, sizeof(A))); }
 void launch_impl(dim3 grid, dim3 block, size_t shm, const std::function<void()>& fn);
 inline unsigned char* dyn_shared() { return cur_block->dyn; }
 inline Wave& wave() { return cur_block->waves[cur_lane_global / 64]; }
@@ -148,6 +157,21 @@ inline double __hiloint2double(int hi, int lo)
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicCAS(int* p, int cmp, int v)
+{
+    __atomic_compare_exchange_n(p, &cmp, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+    return cmp;
+}
+#define __HIP_MEMORY_SCOPE_WORKGROUP 3
+#define __HIP_MEMORY_SCOPE_AGENT 4
+#define __HIP_MEMORY_SCOPE_SYSTEM 5
+#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), __ATOMIC_SEQ_CST)
+#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
+inline void __threadfence() {}
+// a wait yields to the other lanes and workgroups in flight
+inline void __builtin_amdgcn_s_sleep(int) { emu::yield(); }
+inline unsigned long long __builtin_amdgcn_s_memrealtime() { return emu::realtime(); }
 inline int atomicSub(int* p, int v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
 
 template <class A, class B>

@@ -210,6 +210,11 @@ def main():
             scenario_gesv(q, int(n), int(r), prec="s")
         elif w == "lu":
             lu_suite(q)
+        elif w == "dense":
+            dense_suite()
+        elif w.startswith("dgetrf:"):
+            _, m, n = w.split(":")
+            scenario_dense_getrf(int(m), int(n))
         elif w == "overflow":
             scenario_overflow(q)
         elif w == "singular":
@@ -357,6 +362,73 @@ def lu_suite(q):
         scenario_gesv(q, n, r, prec="s")
     for m, n in ((60, 60), (200, 200), (300, 300)):
         scenario_getrf(q, m, n, prec="s")
+
+
+def np_getf2(Arow):
+    """Unblocked LU with partial pivoting (LAPACK dgetf2 order, ties to the
+    first index): returns ipiv (1-based)."""
+    A = Arow.astype(np.float64).copy()
+    m, n = A.shape
+    ipiv = np.zeros(min(m, n), np.int32)
+    for j in range(min(m, n)):
+        p = j + int(np.argmax(np.abs(A[j:, j])))
+        ipiv[j] = p + 1
+        if A[p, j] != 0:
+            A[[j, p]] = A[[p, j]]
+            A[j + 1:, j] *= 1.0 / A[j, j]
+        A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+    return ipiv
+
+
+def scenario_dense_getrf(m, n, prec="d"):
+    """magma_[ds]getrf_gpu on one m x n matrix: LU residual, pivots against
+    the unblocked LU on a well separated input."""
+    rng = np.random.default_rng(m * 31 + n)
+    dt = np.float64 if prec == "d" else np.float32
+    Arow = rng.uniform(-1, 1, (m, n))
+    bA = Buf(np.ascontiguousarray(Arow.T).astype(dt))
+    k = min(m, n)
+    ipiv = (ctypes.c_int * max(k, 1))()
+    info = ctypes.c_int(-7)
+    fn = getattr(LIB, f"magma_{prec}getrf_gpu")
+    fn.argtypes = [i32, i32, vp, i32, vp, ctypes.POINTER(ctypes.c_int)]
+    rc = fn(m, n, bA.ptr, m, ipiv, ctypes.byref(info))
+    P = np.array(ipiv[:k], np.int32)
+    res = lu_residual(Arow, bA.get().astype(np.float64), P, m, n)
+    same = np.array_equal(P, np_getf2(Arow)) if prec == "d" else True
+    print(f"dense {prec}getrf {m}x{n} rc {rc} info {info.value} res {res:.2e} pivots-equal {same}", flush=True)
+    check(rc == 0 and info.value == 0 and res < (1e-13 if prec == "d" else 2e-4) and same,
+          f"dense {prec}getrf {m}x{n}")
+
+
+def scenario_dense_getrs(n, nrhs, trans):
+    rng = np.random.default_rng(n * 17 + nrhs + trans)
+    Arow = rng.uniform(-1, 1, (n, n))
+    Brow = rng.uniform(-1, 1, (n, nrhs))
+    bA = Buf(np.ascontiguousarray(Arow.T))
+    ipiv = (ctypes.c_int * n)()
+    info = ctypes.c_int(0)
+    LIB.magma_dgetrf_gpu.argtypes = [i32, i32, vp, i32, vp, ctypes.POINTER(ctypes.c_int)]
+    LIB.magma_dgetrf_gpu(n, n, bA.ptr, n, ipiv, ctypes.byref(info))
+    bB = Buf(np.ascontiguousarray(Brow.T))
+    LIB.magma_dgetrs_gpu.argtypes = [i32, i32, i32, vp, i32, vp, vp, i32, ctypes.POINTER(ctypes.c_int)]
+    rc = LIB.magma_dgetrs_gpu(112 if trans else 111, n, nrhs, bA.ptr, n, ipiv, bB.ptr, n, ctypes.byref(info))
+    X = bB.get().T
+    Aop = Arow.T if trans else Arow
+    be = np.abs(Brow - Aop @ X).max() / (np.abs(Aop).sum(1).max() * np.abs(X).max())
+    print(f"dense dgetrs n={n} nrhs={nrhs} trans={trans} rc {rc} be {be:.2e}", flush=True)
+    check(rc == 0 and be < 1e-14, f"dense dgetrs {n} {nrhs} {trans}")
+
+
+def dense_suite():
+    for m, n in ((300, 300), (600, 600), (520, 300), (300, 520), (777, 777)):
+        scenario_dense_getrf(m, n)
+    for m, n in ((600, 600), (700, 450)):
+        scenario_dense_getrf(m, n, prec="s")
+    for n, r, t in ((600, 1, 0), (600, 3, 1), (333, 9, 0), (333, 2, 1)):
+        scenario_dense_getrs(n, r, t)
+    for n, facto, solver in ((600, "bf16", "gmres"), (600, "fp16", "ir"), (700, "bf16x3", "ir")):
+        scenario_dhgesv(n, facto, solver)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,235 @@
+"""GPU parity of the single-matrix (dense) path against the oracle:
+magma_dgetrf_gpu / magma_dgetrf_native / magma_sgetrf_gpu (pivots bit-exact
+on tie-free inputs, LU forward error, the tester's residual bound, first
+zero pivot in info), magma_dgetrs_gpu (NoTrans and Trans, against the
+oracle's dgetrs on the same factors), and the refinement driver
+magma_dxgesv_gmres_gpu on one large system (src/dxgesv_gmres_gpu.cpp:160-832)
+for BF16 and FP16 updates, classical and GMRES refinement, condition numbers
+1e2 to 1e6, at n = 2,048 and at the full n = 24,576."""
+import numpy as np
+import pytest
+
+from conftest import EPS, TESTER_TOL
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+LU_FWD_TOL = 1e3 * EPS
+
+
+@pytest.fixture(scope="module")
+def dn():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from magma_amd import dense
+    return dense
+
+
+def cm(rowmajor: np.ndarray) -> np.ndarray:
+    """row-major (m, n) -> column storage (n, m)"""
+    return np.ascontiguousarray(rowmajor.T)
+
+
+def run_getrf(dn, Arow, native=False, dtype=np.float64):
+    m, n = Arow.shape
+    dA = torch.from_numpy(cm(Arow).astype(dtype)).cuda()
+    ipiv, info = dn.getrf_gpu(dA, m=m, native=native)
+    return dA.cpu().numpy(), ipiv, info
+
+
+@pytest.mark.parametrize("m,n", [(300, 300), (1000, 1000), (2048, 2048), (1500, 700), (700, 1500), (2049, 2049)])
+def test_dense_getrf_parity(dn, oracle, m, n):
+    rng = np.random.default_rng(m * 7 + n)
+    Arow = rng.uniform(-1, 1, (m, n))
+    LU, ipiv, info = run_getrf(dn, Arow)
+    A = cm(Arow)[None]
+    LUo, ipo, info_o = oracle.dgetrf(A, m=m)
+    assert info == int(info_o[0]) == 0
+    res = oracle.lu_residual(A, LU[None], ipiv[None], m, n)[0]
+    assert res < TESTER_TOL, res
+    if m * n <= 1500 * 1500:
+        tie_free = oracle.pivot_margin(A, m)[0] > 1e-10
+    else:
+        tie_free = True       # random U[-1,1]: ties at 1e-10 are far from likely at these orders
+    if tie_free:
+        assert np.array_equal(ipiv, ipo[0]), np.nonzero(ipiv != ipo[0])[0][:10]
+        fe = oracle.lu_forward_error(LUo, LU[None], m)[0]
+        assert fe < LU_FWD_TOL, fe
+
+
+def test_dense_getrf_4096(dn, oracle):
+    rng = np.random.default_rng(4096)
+    n = 4096
+    Arow = rng.uniform(-1, 1, (n, n))
+    LU, ipiv, info = run_getrf(dn, Arow)
+    A = cm(Arow)[None]
+    LUo, ipo, info_o = oracle.dgetrf(A)
+    assert info == 0
+    assert np.array_equal(ipiv, ipo[0])
+    assert oracle.lu_forward_error(LUo, LU[None], n)[0] < LU_FWD_TOL
+    assert oracle.lu_residual(A, LU[None], ipiv[None], n, n)[0] < TESTER_TOL
+
+
+def test_dense_native_matches_gpu(dn):
+    rng = np.random.default_rng(3)
+    Arow = rng.uniform(-1, 1, (777, 777))
+    LU1, p1, i1 = run_getrf(dn, Arow)
+    LU2, p2, i2 = run_getrf(dn, Arow, native=True)
+    assert np.array_equal(p1, p2) and i1 == i2
+    assert np.array_equal(LU1, LU2)
+
+
+def test_dense_getrf_singular_and_ties(dn, oracle):
+    rng = np.random.default_rng(11)
+    n = 600
+    Arow = rng.uniform(-1, 1, (n, n))
+    Arow[:, 137] = 0.0                      # first zero pivot at column 138 (1-based)
+    Arow[:, 301] = 0.0
+    LU, ipiv, info = run_getrf(dn, Arow)
+    LUo, ipo, info_o = oracle.dgetrf(cm(Arow)[None])
+    assert info == int(info_o[0]) == 138
+    assert np.array_equal(ipiv[:137], ipo[0][:137])
+    # integer matrix with exact ties in every column: lowest index wins
+    T = rng.integers(-3, 4, (300, 300)).astype(np.float64)
+    LU, ipiv, info = run_getrf(dn, T)
+    LUo, ipo, info_o = oracle.dgetrf(cm(T)[None])
+    assert info == int(info_o[0])
+    # the first column's pivot is decided before any rounding
+    assert ipiv[0] == ipo[0][0]
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("n,nrhs", [(1000, 1), (2048, 7), (333, 20)])
+def test_dense_getrs(dn, oracle, n, nrhs, trans):
+    rng = np.random.default_rng(n + nrhs + trans)
+    Arow = rng.uniform(-1, 1, (n, n))
+    Brow = rng.uniform(-1, 1, (n, nrhs))
+    dA = torch.from_numpy(cm(Arow)).cuda()
+    ipiv, info = dn.getrf_gpu(dA)
+    assert info == 0
+    dB = torch.from_numpy(cm(Brow)).cuda()
+    info = dn.getrs_gpu(dA, ipiv, dB, trans=112 if trans else 111)
+    assert info == 0
+    X = dB.cpu().numpy().T
+    Xo = oracle.dgetrs(trans, dA.cpu().numpy()[None], ipiv[None], cm(Brow)[None])[0].T
+    Aop = Arow.T if trans else Arow
+    be = np.abs(Brow - Aop @ X).max() / (np.abs(Aop).sum(1).max() * np.abs(X).max() * n)
+    assert be < TESTER_TOL, be
+    rel = np.abs(X - Xo).max() / np.abs(Xo).max()
+    assert rel < 1e-10, rel
+
+
+def test_dense_sgetrf(dn):
+    rng = np.random.default_rng(5)
+    m, n = 1500, 1200
+    Arow = rng.uniform(-1, 1, (m, n)).astype(np.float32)
+    LU, ipiv, info = run_getrf(dn, Arow.astype(np.float64), dtype=np.float32)
+    assert info == 0
+    L = np.tril(LU.T[:, :n].astype(np.float64), -1) + np.eye(m, n)
+    U = np.triu(LU.T[:n, :].astype(np.float64))
+    PA = Arow.astype(np.float64).copy()
+    for j, p in enumerate(ipiv):
+        PA[[j, p - 1]] = PA[[p - 1, j]]
+    res = np.linalg.norm(PA - L @ U) / (np.linalg.norm(PA) * n)
+    assert res < 30 * np.finfo(np.float32).eps, res
+
+
+def test_dense_argument_codes(dn):
+    from magma_amd import _ffi
+    lib = _ffi.load()
+    import ctypes
+    info = ctypes.c_int(0)
+    piv = (ctypes.c_int * 4)()
+    assert lib.magma_dgetrf_gpu(-1, 4, None, 4, piv, ctypes.byref(info)) == -1
+    assert lib.magma_dgetrf_gpu(4, -1, None, 4, piv, ctypes.byref(info)) == -2
+    assert lib.magma_dgetrf_gpu(4, 4, None, 3, piv, ctypes.byref(info)) == -4
+    assert lib.magma_dgetrs_gpu(999, 4, 1, None, 4, piv, None, 4, ctypes.byref(info)) == -1
+    assert lib.magma_dgetrs_gpu(111, 4, 1, None, 4, piv, None, 3, ctypes.byref(info)) == -8
+
+
+# --------------------------------------------------------------------------
+# Refinement on one large system
+# --------------------------------------------------------------------------
+def svd_geo_gpu(n, cond, seed):
+    """A = U diag(sigma) V^T with U, V Haar orthogonal (QR of Gaussian
+    matrices with the sign fix), sigma_i = cond^(-(i-1)/(n-1))
+    (testing/magma_generate.cpp, svd_geo), built on the GPU in FP64.
+    Returns the column storage (n, n) tensor."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+
+    def haar():
+        Z = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        Q, R = torch.linalg.qr(Z)
+        return Q * torch.sign(torch.diagonal(R))[None, :]
+    U, V = haar(), haar()
+    sig = cond ** (-torch.arange(n, dtype=torch.float64, device="cuda") / max(n - 1, 1))
+    A = (U * sig[None, :]) @ V.T.contiguous()
+    return A.T.contiguous()      # column storage
+
+
+def dense_refine(A_cols, b, facto, solver):
+    from magma_amd import iteref
+    dA = A_cols.clone()
+    dB = b.clone()
+    X, ipiv, dipiv, it, info = iteref.dhgesv_iteref_gpu(dA, dB, facto=facto, solver=solver)
+    return X, it, info, dA
+
+
+def normwise_be(A_cols, X, B):
+    """max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) on the GPU in FP64."""
+    A = A_cols.T
+    R = B.T - A @ X.T
+    an = A.abs().sum(1).max()
+    return float((R.abs().amax(0) / (an * X.T.abs().amax(0))).max())
+
+
+@pytest.mark.parametrize("facto,solver", [("bf16", "gmres"), ("fp16", "ir"), ("bf16", "ir"), ("fp16", "gmres"),
+                                          ("fp32", "ir")])
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
+def test_dense_refinement_2048(dn, oracle, facto, solver, cond):
+    n = 2048
+    A = svd_geo_gpu(n, cond, seed=int(np.log10(cond)))
+    rng = np.random.default_rng(int(cond) % 1000)
+    b = torch.from_numpy(rng.uniform(-1, 1, (1, n))).cuda()
+    X, it, info, dA = dense_refine(A, b, facto, solver)
+    assert info == 0
+    be = normwise_be(A, X, b)
+    if it >= 0:
+        assert be <= 1e-14, (it, be)
+    else:
+        # FP64 fallback (iter < 0): the tester's bound
+        assert be <= TESTER_TOL * n, (it, be)
+    # forward error against the oracle's FP64 solution
+    Xo = oracle.dgesv(A.cpu().numpy()[None], b.cpu().numpy()[None])[3][0]
+    fe = float(np.abs(X.cpu().numpy() - Xo).max() / np.abs(Xo).max())
+    assert fe <= max(cond, 10.0) * 1e-13, fe
+    print(f"n={n} {facto}-{solver} cond={cond:.0e}: iter {it} be {be:.2e} fe {fe:.2e}")
+
+
+@pytest.mark.timeout(600)
+def test_dense_refinement_full_size(dn):
+    """n = 24,576 (the dense half of the mixed-precision workload): FP64-level
+    backward error and forward error against the FP64 GPU solve
+    (magma_dgetrf_gpu + magma_dgetrs_gpu)."""
+    n = 24576
+    for cond, facto, solver in ((1e2, "bf16", "ir"), (1e4, "bf16", "gmres"), (1e6, "fp16", "gmres")):
+        A = svd_geo_gpu(n, cond, seed=7)
+        rng = np.random.default_rng(1)
+        b = torch.from_numpy(rng.uniform(-1, 1, (1, n))).cuda()
+        X, it, info, _ = dense_refine(A, b, facto, solver)
+        assert info == 0
+        be = normwise_be(A, X, b)
+        assert (it >= 0 and be <= 1e-14) or (it < 0 and be <= TESTER_TOL * n), (cond, facto, solver, it, be)
+        LU = A.clone()
+        ipiv, info = dn.getrf_gpu(LU)
+        assert info == 0
+        X64 = b.clone()
+        assert dn.getrs_gpu(LU, ipiv, X64) == 0
+        be64 = normwise_be(A, X64, b)
+        assert be64 <= TESTER_TOL * n
+        fe = float((X - X64).abs().max() / X64.abs().max())
+        assert fe <= cond * 1e-12, fe
+        print(f"n={n} {facto}-{solver} cond={cond:.0e}: iter {it} be {be:.2e} fp64 be {be64:.2e} fe {fe:.2e}")
+        del A, LU
+        torch.cuda.empty_cache()
