@@ -40,7 +40,9 @@ __device__ __forceinline__ int ld_agent_i(const int* p)
 // KIND 0: L (unit lower), forward; 1: U, backward; 2: U^T (lower), forward;
 // 3: L^T (upper, unit), backward. NR right-hand sides per workgroup.
 // Thread t works on row t % 128 of its block and on half t / 128 of the
-// columns of each off-diagonal block.
+// columns of each off-diagonal block. Its 64 matrix values of a block do not
+// depend on the solution, so they are loaded before the block's flag is
+// awaited (one memory latency per block instead of one per column).
 template <typename T, int KIND, int NR>
 __global__ void __launch_bounds__(256)
 dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __restrict__ invD,
@@ -49,6 +51,7 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
 {
     constexpr bool kFwd = (KIND == 0 || KIND == 2);
     constexpr bool kTr = (KIND == 2 || KIND == 3);
+    constexpr int H = BS / 2;
     __shared__ T xs[BS][NR];
     __shared__ T red[BS][NR];
     __shared__ int s_tk;
@@ -77,6 +80,15 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
         const int c = kFwd ? d : nblk - 1 - d;
         const int c0 = c * BS;
         const int cn = min(BS, n - c0);
+        // this thread's part of block (b, c), independent of x_c
+        T mv[H];
+#pragma unroll
+        for (int jj = 0; jj < H; ++jj) {
+            const int col = h * H + jj;
+            mv[jj] = (rr < nb && col < cn) ? (kTr ? LU[(size_t)(c0 + col) + (size_t)(r0 + rr) * ld]
+                                                  : LU[(size_t)(r0 + rr) + (size_t)(c0 + col) * ld])
+                                           : T(0);
+        }
         if (t == 0) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             while (ld_agent_i(&fl[c]) != epoch) {
@@ -95,17 +107,21 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
             xs[i][q] = (i < cn && q < nq) ? X[(size_t)(c0 + i) + (size_t)(q0 + q) * ldx] : T(0);
         }
         __syncthreads();
-        if (rr < nb) {
-            const int jbeg = h * (BS / 2), jend = min(cn, jbeg + BS / 2);
-            for (int jj = jbeg; jj < jend; ++jj) {
-                const T a = kTr ? LU[(size_t)(c0 + jj) + (size_t)(r0 + rr) * ld]
-                                : LU[(size_t)(r0 + rr) + (size_t)(c0 + jj) * ld];
 #pragma unroll
-                for (int q = 0; q < NR; ++q) acc[q] = fma(-a, xs[jj][q], acc[q]);
-            }
+        for (int jj = 0; jj < H; ++jj) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = fma(-mv[jj], xs[h * H + jj][q], acc[q]);
         }
     }
     // y = sum of the two halves; x_b = D^{-1} y
+    const T* D = invD + (size_t)b * BS * BS;
+    T dv[H];
+#pragma unroll
+    for (int jj = 0; jj < H; ++jj) {
+        const int col = h * H + jj;
+        dv[jj] = (rr < nb && col < nb) ? (kTr ? D[(size_t)col + (size_t)rr * BS] : D[(size_t)rr + (size_t)col * BS])
+                                       : T(0);
+    }
     __syncthreads();
     if (h == 1) {
 #pragma unroll
@@ -117,17 +133,13 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
         for (int q = 0; q < NR; ++q) xs[rr][q] = (rr < nb) ? acc[q] + red[rr][q] : T(0);
     }
     __syncthreads();
-    const T* D = invD + (size_t)b * BS * BS;
     T out[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) out[q] = T(0);
-    if (rr < nb) {
-        const int jbeg = h * (BS / 2), jend = min(nb, jbeg + BS / 2);
-        for (int jj = jbeg; jj < jend; ++jj) {
-            const T a = kTr ? D[(size_t)jj + (size_t)rr * BS] : D[(size_t)rr + (size_t)jj * BS];
 #pragma unroll
-            for (int q = 0; q < NR; ++q) out[q] = fma(a, xs[jj][q], out[q]);
-        }
+    for (int jj = 0; jj < H; ++jj) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) out[q] = fma(dv[jj], xs[h * H + jj][q], out[q]);
     }
     if (h == 1) {
 #pragma unroll
@@ -234,8 +246,6 @@ magma_int_t dense_set_perm(int n, const int* h_ipiv, const SolveWork& s, hipStre
 
 namespace {
 
-int g_solve_epoch = 0;
-
 template <typename T, int NR>
 magma_int_t trsv_pair(int trans, int n, int nrhs, const T* LU, int ld, T* Y, int ldy, const SolveWork& s,
                       hipStream_t st)
@@ -247,22 +257,19 @@ magma_int_t trsv_pair(int trans, int n, int nrhs, const T* LU, int ld, T* Y, int
     const T* iU = reinterpret_cast<const T*>(s.invU);
     dense_solve_reset_kernel<<<(unsigned)ceildiv(nwg, 256), 256, 0, st>>>(s.flags, nwg, s.ticket, s.err);
     MGI_LAUNCH_CHECK();
-    int e1 = ++g_solve_epoch;
+    const int e1 = 1, e2 = 2;     // the flags are cleared for every pair of solves
     if (trans == 0) {
         dense_trsv_kernel<T, 0, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.flags, s.ticket, s.err, e1);
         MGI_LAUNCH_CHECK();
         dense_ticket_reset_kernel<<<1, 1, 0, st>>>(s.ticket);
-        int e2 = ++g_solve_epoch;
         dense_trsv_kernel<T, 1, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.flags, s.ticket, s.err, e2);
     } else {
         dense_trsv_kernel<T, 2, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.flags, s.ticket, s.err, e1);
         MGI_LAUNCH_CHECK();
         dense_ticket_reset_kernel<<<1, 1, 0, st>>>(s.ticket);
-        int e2 = ++g_solve_epoch;
         dense_trsv_kernel<T, 3, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.flags, s.ticket, s.err, e2);
     }
     MGI_LAUNCH_CHECK();
-    if (g_solve_epoch > (1 << 29)) g_solve_epoch = 0;
     return MAGMA_SUCCESS;
 }
 

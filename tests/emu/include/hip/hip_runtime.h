@@ -287,6 +287,8 @@ inline hipError_t hipDeviceGetAttribute(int* v, int attr, int)
     return 0;
 }
 inline hipError_t hipFuncSetAttribute(const void*, int, int) { return 0; }
+// a __device__ array is a host array here: its address is the symbol's
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }
 inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)0x1; return 0; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)0x1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }

@@ -84,10 +84,12 @@ def test_iteref_batched_inputs_unchanged(ir):
     assert np.array_equal(dA.cpu().numpy(), A) and np.array_equal(dB.cpu().numpy(), B)
 
 
-@pytest.mark.skip(reason="an earlier build faulted the GPU (illegal address) on this input, when the FP32 LU "
-                         "still ran on the overflowed copy; no out-of-bounds access was found by reading, so the "
-                         "case stays off the GPU until a cause is shown")
 def test_iteref_overflow_falls_back(ir, oracle):
+    """Down-cast overflow (src/dxgesv_gmres_gpu.cpp:403-410): the system whose
+    FP32 copy overflows skips the low-precision LU and is solved in FP64
+    (iter = -2); the others refine as usual. (An earlier build ran the FP32 LU
+    on the infinite copy without the first-NaN pivot rule: an all-NaN column
+    left the pivot index unset and the row interchanges read a wild row.)"""
     rng = np.random.default_rng(2)
     n = 40
     A, B = batch_of(rng, 3, n, 10.0, 1)
@@ -100,6 +102,36 @@ def test_iteref_overflow_falls_back(ir, oracle):
     assert it[1] == -2 and it[0] >= 0 and it[2] >= 0, it
     be = oracle.gesv_backward_error(A, X.cpu().numpy(), B, n)
     assert be.max() < TESTER_TOL
+
+
+@pytest.mark.parametrize("facto", ["bf16", "fp16"])
+@pytest.mark.parametrize("solver", ["ir", "gmres"])
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
+def test_iteref_batched_n256(ir, oracle, facto, solver, cond):
+    """The batch half of the mixed-precision workload at n = 256 (a few hundred
+    svd_geo systems per condition number): every system either reaches FP64
+    backward error (<= 1e-14) with the forward error bounded against the
+    oracle's dgesv, or reports a negative iter and meets the tester's bound
+    through the FP64 fallback."""
+    rng = np.random.default_rng(int(np.log10(cond)) * 10 + len(facto) + len(solver))
+    batch, n, nrhs = 200, 256, 1
+    A, B = batch_of(rng, batch, n, cond, nrhs)
+    dA = torch.from_numpy(A.copy()).cuda()
+    dB = torch.from_numpy(B.copy()).cuda()
+    X, it, info = ir.dxgesv_iteref_batched(dA, dB, facto=facto, solver=solver, itermax=100)
+    torch.cuda.synchronize()
+    it = it.cpu().numpy()
+    X = X.cpu().numpy()
+    assert (info.cpu().numpy() == 0).all()
+    be = backward_errors(oracle, A, X, B, n)
+    ok = it >= 0
+    assert (be[ok] <= 1e-14).all(), be[ok].max()
+    assert (be[~ok] <= TESTER_TOL * n).all()
+    Xo = oracle.dgesv(A, B)[3]
+    fe = np.abs(X - Xo).max(axis=(1, 2)) / np.abs(Xo).max(axis=(1, 2))
+    assert fe.max() <= max(cond, 10.0) * 1e-13, fe.max()
+    print(f"{facto}-{solver} cond {cond:.0e}: converged {ok.sum()}/{batch}, iter max {it.max()}, "
+          f"be max {be.max():.2e}, fe max {fe.max():.2e}")
 
 
 def test_iteref_singular_reports_info(ir):

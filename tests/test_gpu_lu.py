@@ -60,7 +60,8 @@ def test_getrf_square_random(mb, oracle, n):
 
 
 @pytest.mark.parametrize("m,n", [(129, 129), (150, 150), (200, 200), (256, 256), (300, 300), (512, 512),
-                                 (300, 200), (200, 300), (700, 700), (1100, 1100)])
+                                 (300, 200), (200, 300), (700, 700), (1100, 1100),
+                                 (300, 400), (290, 600), (480, 300), (449, 449)])
 def test_getrf_blocked_sizes(mb, oracle, m, n):
     """Sizes beyond one workgroup's LDS take the blocked panel/laswp/trsm/gemm
     path."""
@@ -160,6 +161,60 @@ def test_gesv_batched(mb, oracle, n, nrhs):
     LUo, ipo, _ = oracle.dgetrf(A)
     tf = oracle.pivot_margin(A) > 1e-10
     assert np.array_equal(ipiv.cpu().numpy()[tf], ipo[tf])
+
+
+@pytest.mark.parametrize("n,nrhs", [(129, 3), (200, 1), (256, 4), (300, 2), (449, 17)])
+def test_gesv_batched_blocked(mb, oracle, n, nrhs):
+    """n > 128: magma_dgesv_batched as the blocked getrf + getrs composite,
+    against the oracle's dgesv (pivots on tie-free systems, solution and
+    backward error)."""
+    rng = np.random.default_rng(900 + n)
+    batch = 24
+    A = uniform_batch(rng, batch, n)
+    B = uniform_batch(rng, batch, n, cols=nrhs)
+    dA, dB = to_dev(A), to_dev(B)
+    ipiv, info = mb.dgesv_batched(dA, dB)
+    torch.cuda.synchronize()
+    X = dB.cpu().numpy()
+    assert (info.cpu().numpy() == 0).all()
+    be = oracle.gesv_backward_error(A, X, B, n)
+    assert (be < TESTER_TOL).all(), be.max()
+    LUo, ipo, info_o, Xo = oracle.dgesv(A, B)
+    tf = oracle.pivot_margin(A) > 1e-10
+    assert np.array_equal(ipiv.cpu().numpy()[tf], ipo[tf])
+    rel = np.abs(X - Xo).max(axis=(1, 2)) / np.abs(Xo).max(axis=(1, 2))
+    assert rel.max() < 1e-9, rel.max()
+
+
+def test_batch_beyond_grid_limit(mb, oracle):
+    """70,000 systems of order 150 (more than the 65,535 matrices one grid y/z
+    extent holds): magma_dgetrf_batched + magma_dgetrs_batched split the
+    launches; every system's backward error is checked on the GPU and a
+    sample against the oracle, including systems past the first split."""
+    batch, n, nrhs = 70000, 150, 1
+    g = torch.Generator(device="cuda").manual_seed(77)
+    dA = torch.rand((batch, n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    dB = torch.rand((batch, nrhs, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    A0, B0 = dA.clone(), dB.clone()
+    ipiv, info = mb.dgetrf_batched(dA)
+    mb.dgetrs_batched(111, dA, ipiv, dB)
+    torch.cuda.synchronize()
+    assert int((info != 0).sum()) == 0
+    be = torch.empty(batch, dtype=torch.float64, device="cuda")
+    for c in range(0, batch, 10000):
+        Ar = A0[c:c + 10000].transpose(1, 2).contiguous()
+        Xr = dB[c:c + 10000].transpose(1, 2).contiguous()
+        Br = B0[c:c + 10000].transpose(1, 2).contiguous()
+        R = Br - torch.bmm(Ar, Xr)
+        be[c:c + 10000] = R.abs().sum(2).amax(1) / (n * Ar.abs().sum(2).amax(1) * Xr.abs().sum(2).amax(1))
+    assert float(be.max()) < TESTER_TOL
+    sample = np.array([0, 1, 65534, 65535, 65536, 69999])
+    A_s = A0[sample].cpu().numpy()
+    LUo, ipo, _ = oracle.dgetrf(A_s)
+    tf = oracle.pivot_margin(A_s) > 1e-10
+    assert np.array_equal(ipiv.cpu().numpy()[sample][tf], ipo[tf])
+    del dA, dB, A0, B0
+    torch.cuda.empty_cache()
 
 
 def test_gesv_config2_full_size(mb, oracle):
