@@ -354,6 +354,19 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs,
         magma_refinement_t facto_type, magma_refinement_t solver_type, magma_int_t itermax,
         magma_int_t *iter_array, magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
 
+/* No MAGMA counterpart. For callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the LU of magma_dgetrf_native (same kernels, same results),
+ * with its scratch in a fixed 512 KB device array of the library instead of
+ * allocations. d_ipiv (min(m,n) entries) and d_info are device memory; the
+ * call is asynchronous on queue's stream. m <= 256 * (compute units, at most
+ * 256), i.e. 65,536 rows on MI355X: larger m returns MAGMA_ERR_NOT_SUPPORTED.
+ * Calls that overlap on one device share that array and race: serialise them
+ * (one queue, or events between queues). A wait that exceeds its time limit
+ * sets *d_info = MAGMA_ERR_UNKNOWN. */
+magma_int_t magma_dgetrf_native_noalloc(magma_int_t m, magma_int_t n, magmaDouble_ptr dA, magma_int_t ldda,
+                                        magmaInt_ptr d_ipiv, magmaInt_ptr d_info, magma_queue_t queue);
+
 #ifdef __cplusplus
 }
 #endif

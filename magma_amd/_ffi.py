@@ -106,6 +106,7 @@ _PROTOS = {
     "magma_xshgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p, c_int, c_int]),
     "magma_sgetrf_prec_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p, c_int]),
     "magma_sgetrs_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_int, i32p]),
+    "magma_dgetrf_native_noalloc": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "magma_dxgesv_iteref_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
                                             c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]),
 }
