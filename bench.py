@@ -79,6 +79,10 @@ def parse():
                     help="also run the dense mixed-precision leg (one system of order 24,576 at three condition "
                          "numbers); 0 skips")
     ap.add_argument("--dense-n", type=int, default=24576)
+    ap.add_argument("--dense-conds", default="1e2,1e4,1e6",
+                    help="condition numbers of the dense leg, comma separated")
+    ap.add_argument("--dense-variants", default="bf16:ir,bf16:gmres,fp16:ir,fp16:gmres,bf16x3:ir,bf16x3:gmres",
+                    help="facto:solver pairs of the dense leg, comma separated")
     ap.add_argument("--c1", type=int, default=1,
                     help="also run the small CPU-path leg (1,000 x 32^2, host LAPACK beside the GPU); 0 skips")
     ap.add_argument("--vbatched", type=int, default=1,
@@ -275,7 +279,8 @@ def haar_cols(torch, n, gen, dev):
 
 
 def dense_leg(torch, dev, seed, n=24576, conds=(1e2, 1e4, 1e6),
-              variants=(("bf16", "ir"), ("bf16", "gmres"), ("fp16", "ir"), ("fp16", "gmres"))):
+              variants=(("bf16", "ir"), ("bf16", "gmres"), ("fp16", "ir"), ("fp16", "gmres"), ("bf16x3", "ir"),
+                        ("bf16x3", "gmres"))):
     """The dense half of the mixed-precision workload: one system of order n,
     A = U diag(sigma) V^T (Haar U and V, generated once; sigma geometric to
     cond), b ~ U[-1,1]. FP64 magma_dgetrf_gpu + magma_dgetrs_gpu against the
@@ -305,7 +310,7 @@ def dense_leg(torch, dev, seed, n=24576, conds=(1e2, 1e4, 1e6),
     # low-precision LU alone (MFMA fraction); the matrix does not matter for its speed
     Af = (U * 0.5).T.contiguous().float()
     lu_only = {}
-    for facto, peak in (("bf16", 2500.0), ("fp16", 2500.0)):
+    for facto, peak in (("bf16", 2500.0), ("fp16", 2500.0), ("bf16x3", 2500.0 / 3)):
         W = Af.clone()
         (ipiv, info), t = timed(lambda: dense.sgetrf_prec_gpu(W, facto))
         lu_only[facto] = {"ms": round(t * 1e3, 2), "tflops": round(flops_getrf(n) / t / 1e12, 2),
@@ -607,7 +612,9 @@ def main():
     dense_res = None
     if args.dense:
         print("[bench] dense mixed-precision leg", file=sys.stderr, flush=True)
-        dense_res = dense_leg(torch, dev, args.seed, n=args.dense_n)
+        dense_res = dense_leg(torch, dev, args.seed, n=args.dense_n,
+                              conds=tuple(float(c) for c in args.dense_conds.split(",") if c),
+                              variants=tuple(tuple(v.split(":")) for v in args.dense_variants.split(",") if v))
     c1 = None
     if args.c1:
         print("[bench] small CPU-path leg", file=sys.stderr, flush=True)

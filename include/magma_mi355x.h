@@ -272,6 +272,40 @@ void magma_sset_pointer(float **output_array, float *input, magma_int_t lda,
                         magma_int_t batchCount, magma_queue_t queue);
 
 /* ------------------------------------------------------------------------- */
+/* Batched LU without pivoting, random butterflies, batched inverse            */
+/* ------------------------------------------------------------------------- */
+/* include/magma_zbatched.h:977 (src/zgetrf_nopiv_batched.cpp), d form: LU with
+ * the diagonal as pivots; info_array[i] = first exactly-zero pivot (1-based).
+ * m <= 2048. Host-synchronous. */
+magma_int_t magma_dgetrf_nopiv_batched(magma_int_t m, magma_int_t n, double **dA_array, magma_int_t ldda,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+/* magma_zbatched.h (magma_zgetrs_nopiv_batched): solve with those factors. */
+magma_int_t magma_dgetrs_nopiv_batched(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+                                       double **dA_array, magma_int_t ldda, double **dB_array, magma_int_t lddb,
+                                       magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+/* magma_zbatched.h (magma_zgesv_nopiv_batched): factor and solve. */
+magma_int_t magma_dgesv_nopiv_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                      double **dB_array, magma_int_t lddb, magma_int_t *info_array,
+                                      magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zvbatched.h:218 (src/zgetrf_nopiv_vbatched.cpp): m, n, ldda are
+ * device arrays; max(m) <= 2048. Host-synchronous. */
+magma_int_t magma_dgetrf_nopiv_vbatched(magma_int_t *m, magma_int_t *n, double **dA_array, magma_int_t *ldda,
+                                        magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zbatched.h:1001 (src/zgesv_rbt_batched.cpp): A := U^T A V and
+ * B := U^T B with two-level random butterflies (entries exp((r-1/2)/10),
+ * r ~ U[0,1] from a fixed seed), LU without pivoting, solve, B := V B.
+ * A is overwritten by the factors of U^T A V. Host-synchronous. */
+magma_int_t magma_dgesv_rbt_batched(magma_int_t n, magma_int_t nrhs, double **dA_array, magma_int_t ldda,
+                                    double **dB_array, magma_int_t lddb, magma_int_t *dinfo_array,
+                                    magma_int_t batchCount, magma_queue_t queue);
+/* include/magma_zbatched.h:872 (src/zgetri_outofplace_batched.cpp): inverse from
+ * the factors and pivots of magma_dgetrf_batched, into dinvA_array (lddia).
+ * Host-synchronous. */
+magma_int_t magma_dgetri_outofplace_batched(magma_int_t n, double **dA_array, magma_int_t ldda,
+                                            magma_int_t **dipiv_array, double **dinvA_array, magma_int_t lddia,
+                                            magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
+
+/* ------------------------------------------------------------------------- */
 /* One large matrix on the whole device (dense LU)                            */
 /* ------------------------------------------------------------------------- */
 /* include/magma_z.h:612 (src/zgetrf_gpu.cpp:542), d form: LU with partial
@@ -322,9 +356,22 @@ magma_int_t magma_dhgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
         magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
         magmaDouble_ptr dworkd, magmaFloat_ptr dworks, magma_int_t *iter, magma_int_t *info);
 
+/* src/dxgesv_gmres_gpu.cpp:966 -- FP32 LU (FP32 updates) + FGMRES in FP64
+ * preconditioned by it (Magma_PREC_SS, Magma_REFINE_GMSTRS); same contract as
+ * magma_dhgesv_iteref_gpu. */
+magma_int_t magma_dsgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
+        magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv, magmaInt_ptr dipiv,
+        magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
+        magmaDouble_ptr dworkd, magmaFloat_ptr dworks, magma_int_t *iter, magma_int_t *info);
+
 /* src/dxgesv_gmres_gpu.cpp:160 -- expert driver: facto_type in {SS, SHT,
- * XSH_163232TC (same as SHT), SBT, SB3T}, solver_type in {IRSTRS, GMSTRS}.
- * facto_time (seconds) may be NULL. */
+ * XSH_163232TC (same as SHT), SBT, SB3T}, solver_type any of the eight
+ * Magma_REFINE_* values (:328-359): IR or FGMRES outer level, optionally an
+ * inner FGMRES (50 iterations to 1e-4) as preconditioner, with the
+ * triangular solves on the FP32 factors (..STRS) or on their FP64 copy
+ * (..DTRS). The outer FGMRES restarts after min(n, 500) iterations. Every
+ * mode that does not reach ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)
+ * falls back to FP64 (iter < 0). facto_time (seconds) may be NULL. */
 magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs,
         magmaDouble_ptr dA, magma_int_t ldda, magma_int_t *ipiv, magmaInt_ptr dipiv,
         magmaDouble_ptr dB, magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx,
@@ -357,7 +404,7 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs,
 /* No MAGMA counterpart. For callers that cannot allocate device memory (code
  * under hipGraph capture, tight serving loops, harnesses that forbid
  * hipMalloc): the LU of magma_dgetrf_native (same kernels, same results),
- * with its scratch in a fixed 512 KB device array of the library instead of
+ * with its scratch in a fixed 1 MB device array of the library instead of
  * allocations. d_ipiv (min(m,n) entries) and d_info are device memory; the
  * call is asynchronous on queue's stream. m <= 256 * (compute units, at most
  * 256), i.e. 65,536 rows on MI355X: larger m returns MAGMA_ERR_NOT_SUPPORTED.

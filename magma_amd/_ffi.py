@@ -24,7 +24,8 @@ i32p = ctypes.POINTER(ctypes.c_int)
 MagmaNoTrans, MagmaTrans, MagmaConjTrans = 111, 112, 113
 Magma_PREC_SS, Magma_PREC_SHT, Magma_PREC_XSH_163232TC = 900, 905, 917
 Magma_PREC_SBT, Magma_PREC_SB3T = 940, 941
-Magma_REFINE_IRSTRS, Magma_REFINE_GMSTRS = 920, 924
+(Magma_REFINE_IRSTRS, Magma_REFINE_IRDTRS, Magma_REFINE_IRGMSTRS, Magma_REFINE_IRGMDTRS, Magma_REFINE_GMSTRS,
+ Magma_REFINE_GMDTRS, Magma_REFINE_GMGMSTRS, Magma_REFINE_GMGMDTRS) = range(920, 928)
 Magma_MP_ENABLE_DFLT_MATH, Magma_MP_ENABLE_TC_MATH = 960, 961
 Magma_MP_SGEMM, Magma_MP_GEMEX_I16_O32_C32 = 962, 965
 MAGMA_ERR_NOT_SUPPORTED = -103
@@ -96,9 +97,21 @@ _PROTOS = {
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
     "magma_dbgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
+    "magma_dsgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
+                                        c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
     "magma_dxgesv_gmres_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                        c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, i32p, i32p,
                                        ctypes.POINTER(c_double)]),
+    "magma_dgetrf_nopiv_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "magma_dgetrs_nopiv_batched": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p,
+                                           c_int, c_void_p]),
+    "magma_dgesv_nopiv_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
+                                          c_void_p]),
+    "magma_dgetrf_nopiv_vbatched": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "magma_dgesv_rbt_batched": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
+                                        c_void_p]),
+    "magma_dgetri_outofplace_batched": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                                c_int, c_void_p]),
     "magma_dgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),
     "magma_dgetrf_native": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),
     "magma_sgetrf_gpu": (c_int, [c_int, c_int, c_void_p, c_int, i32p, i32p]),

@@ -406,3 +406,67 @@ class VbatchedPlan:
             self.close()
         except Exception:
             pass
+
+
+def dgetrf_nopiv_batched(A: torch.Tensor, m: int | None = None, queue: Queue | None = None):
+    """In-place LU without pivoting of every matrix of A (batch, n, ldda).
+    Returns info (batch,) int32 (first exactly-zero pivot, 1-based)."""
+    _check_batch(A, "A")
+    batch, n, ldda = A.shape
+    m = ldda if m is None else m
+    queue = queue or default_queue(A.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
+    pA = pointer_array(A, queue)
+    rc = _ffi.load().magma_dgetrf_nopiv_batched(m, n, pA.data_ptr(), ldda, info.data_ptr(), batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgetrf_nopiv_batched", rc)
+    return info
+
+
+def dgesv_nopiv_batched(A: torch.Tensor, B: torch.Tensor, queue: Queue | None = None):
+    """A_s X_s = B_s with the LU without pivoting (A -> LU, B -> X). Returns info."""
+    _check_batch(A, "A")
+    _check_batch(B, "B")
+    batch, n, ldda = A.shape
+    _, nrhs, lddb = B.shape
+    queue = queue or default_queue(A.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
+    pA, pB = pointer_array(A, queue), pointer_array(B, queue)
+    rc = _ffi.load().magma_dgesv_nopiv_batched(n, nrhs, pA.data_ptr(), ldda, pB.data_ptr(), lddb, info.data_ptr(),
+                                               batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgesv_nopiv_batched", rc)
+    return info
+
+
+def dgesv_rbt_batched(A: torch.Tensor, B: torch.Tensor, queue: Queue | None = None):
+    """A_s X_s = B_s through random butterflies and the LU without pivoting
+    (magma_dgesv_rbt_batched). B -> X. Returns info."""
+    _check_batch(A, "A")
+    _check_batch(B, "B")
+    batch, n, ldda = A.shape
+    _, nrhs, lddb = B.shape
+    queue = queue or default_queue(A.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
+    pA, pB = pointer_array(A, queue), pointer_array(B, queue)
+    rc = _ffi.load().magma_dgesv_rbt_batched(n, nrhs, pA.data_ptr(), ldda, pB.data_ptr(), lddb, info.data_ptr(),
+                                             batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgesv_rbt_batched", rc)
+    return info
+
+
+def dgetri_outofplace_batched(LU: torch.Tensor, ipiv: torch.Tensor, queue: Queue | None = None) -> torch.Tensor:
+    """Inverse of every matrix from its LU factors and pivots
+    (magma_dgetri_outofplace_batched). Returns a new (batch, n, n) tensor."""
+    _check_batch(LU, "LU")
+    batch, n, ldda = LU.shape
+    queue = queue or default_queue(LU.device)
+    inv = torch.empty((batch, n, n), dtype=torch.float64, device=LU.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=LU.device)
+    pA, pP, pI = pointer_array(LU, queue), pointer_array(_ipiv_view(ipiv), queue), pointer_array(inv, queue)
+    rc = _ffi.load().magma_dgetri_outofplace_batched(n, pA.data_ptr(), ldda, pP.data_ptr(), pI.data_ptr(), n,
+                                                     info.data_ptr(), batch, queue.handle)
+    if rc < 0:
+        raise MagmaError("magma_dgetri_outofplace_batched", rc)
+    return inv

@@ -7,13 +7,26 @@
 //   2. the FP32 copy is factored by the dense LU with FP32, BF16, FP16 or
 //      split-BF16 trailing updates -- a zero pivot or a non-finite factor:
 //      iter = -3, FP64 fallback;
-//   3. IRSTRS: x = LU^{-1} b, then x += LU^{-1} r with r = b - A x in FP64
-//      until ||r_j||_inf <= ||x_j||_inf * cte for every column,
-//      cte = ||A||_inf * eps * sqrt(n) (:490-493, :648, :755);
-//      GMSTRS: FGMRES (restart 50) right-preconditioned by the FP32 LU, per
-//      column, until ||r||_2 <= cte * ||r0||_2 (src/dgmres_plu_gpu.cpp);
-//   4. no convergence within ITERMAX = min(n, 500) steps: FP64 LU and
-//      solve of the original system (:801-810), iter = -steps.
+//   3. refinement, with FP64 residuals r = b - A x and the test
+//      ||r_j||_inf <= ||x_j||_inf * cte, cte = ||A||_inf * eps * sqrt(n)
+//      (:490-493, :648, :755). The correction solve applies the factors
+//      in FP32 (..STRS, magma_dsgetrs_gpu) or their FP64 copy (..DTRS,
+//      magma_dgetrs_gpu on slag2d of them, :445-452):
+//        IR*TRS    x = LU^{-1} b, then x += LU^{-1} r;
+//        IRGM*TRS  x += d with d from an inner FGMRES on A d = r
+//                  (50 iterations to 1e-4, preconditioned by the solve);
+//        GM*TRS    FGMRES right-preconditioned by the solve, per column,
+//                  restart = ITERMAX (:574-575), until ||r||_2 <= cte
+//                  ||r0||_2 (src/dgmres_plu_gpu.cpp) or the test above
+//                  holds for the true residual (MAGMA's test alone cannot
+//                  be met in FP64 when ||x|| is large);
+//        GMGM*TRS  the same with the inner FGMRES as preconditioner;
+//      ITERMAX = min(n, 500); with an inner FGMRES the outer level takes
+//      max(10, ITERMAX / 50) steps (:211-219, 328-359);
+//   4. no convergence: FP64 LU and solve of the original system
+//      (:801-810), iter = -steps. (MAGMA's FGMRES modes return without this
+//      check; here every mode falls back, so the answer is always FP64
+//      quality.)
 // The host drives the iterations; vectors, residuals and the Krylov basis
 // stay in device memory, and only norms and the Hessenberg column come back.
 #include "magma_internal.h"
@@ -36,7 +49,7 @@ magma_int_t dense_solve_check_err(const SolveWork& s, hipStream_t stream);
 namespace {
 
 constexpr int kRT = 256;          // threads of the vector kernels
-constexpr int kGmresRestart = 50; // inner_itermax of src/dxgesv_gmres_gpu.cpp:212
+constexpr int kGmresInner = 50;   // inner_itermax of src/dxgesv_gmres_gpu.cpp:212
 
 // --------------------------------------------------------------------------
 // FP64 -> FP32 copy with the overflow flag and partial row sums of |a| over
@@ -213,18 +226,20 @@ __global__ void dn_sum_parts_kernel(int n, int nchunk, const double* __restrict_
     out[row] = s;
 }
 
-// FP64 -> FP32 with the row map (NoTrans) or without (Trans)
-__global__ void dn_gather_d2s_kernel(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ S,
-                                     int lds, float* __restrict__ D, int ldd)
+// FP64 -> TD with the row map (NoTrans) or without (Trans)
+template <typename TD>
+__global__ void dn_gather_kernel(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ S, int lds,
+                                 TD* __restrict__ D, int ldd)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
-    if (i < n && q < nrhs) D[(size_t)i + (size_t)q * ldd] = (float)S[(size_t)(perm ? perm[i] : i) + (size_t)q * lds];
+    if (i < n && q < nrhs) D[(size_t)i + (size_t)q * ldd] = (TD)S[(size_t)(perm ? perm[i] : i) + (size_t)q * lds];
 }
 
 // X[dest(i)] (+)= Y[i]: dest = i (NoTrans) or perm[i] (Trans); acc = 0 stores.
-__global__ void dn_apply_s2d_kernel(int n, int nrhs, const int* __restrict__ perm, const float* __restrict__ Y,
-                                    int ldy, double* __restrict__ X, int ldx, int acc)
+template <typename TS>
+__global__ void dn_apply_kernel(int n, int nrhs, const int* __restrict__ perm, const TS* __restrict__ Y, int ldy,
+                                double* __restrict__ X, int ldx, int acc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int q = blockIdx.y;
@@ -233,6 +248,14 @@ __global__ void dn_apply_s2d_kernel(int n, int nrhs, const int* __restrict__ per
         const double y = (double)Y[(size_t)i + (size_t)q * ldy];
         *p = acc ? *p + y : y;
     }
+}
+
+// FP32 factors -> FP64 copy, ld n (magmablas_slag2d)
+__global__ void dn_lag2d_kernel(int n, const float* __restrict__ S, int lds, double* __restrict__ D)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n) D[(size_t)i + (size_t)j * n] = (double)S[(size_t)i + (size_t)j * lds];
 }
 
 __global__ void dn_copy_d_kernel(int n, int nrhs, const double* __restrict__ S, int lds, double* __restrict__ D,
@@ -362,10 +385,10 @@ magma_int_t getrf_one(int mode, int m, int n, T* dA, int ldda, int* h_ipiv, int*
     const int gmax = std::max(dense_panel_gmax(m), 1);
     if (gmax > device_cu_count()) return MAGMA_ERR_NOT_SUPPORTED;
     DevBuf work, small;
-    MGI_HIP_CHECK(work.alloc(dense_work_bytes(n, (int)sizeof(T), gmax)));
+    MGI_HIP_CHECK(work.alloc(dense_work_bytes(m, n, (int)sizeof(T), gmax)));
     MGI_HIP_CHECK(small.alloc(sizeof(int) * (minmn + 64)));
     DenseWork w;
-    dense_work_carve(work.p, n, (int)sizeof(T), gmax, w);
+    dense_work_carve(work.p, m, n, (int)sizeof(T), gmax, w);
     int* d_info = static_cast<int*>(small.p);
     int* d_ipiv = d_info + 64;
     magma_int_t rc;
@@ -450,27 +473,29 @@ struct DenseIrCtx {
     float* S; int lds;          // FP32 LU
     float* Ys;                  // FP32 staging n x nrhs
     SolveWork sw;
-    double* part; double* nrm; int nblk_rows;
+    bool dtrs;                  // correction solves on the FP64 copy of the factors
+    double* Dl;                 // that copy, ld n
+    double* Yd;                 // FP64 staging n x nrhs
+    SolveWork swd;
+    double* part; double* nrm;
     double cte;
     hipStream_t st;
 };
 
-// R = B - op(A) X; returns per-column ||R||_inf and ||X||_inf.
-static magma_int_t ir_residual(DenseIrCtx& c, int q0, int nq, std::vector<double>& rn, std::vector<double>& xn)
+// R = B - op(A) X for nq columns; per-column ||R||_inf and ||X||_inf.
+static magma_int_t ir_residual(DenseIrCtx& c, const double* B, int ldb, const double* X, int ldx, double* R, int ldr,
+                               int nq, std::vector<double>& rn, std::vector<double>& xn)
 {
     const int n = c.n;
-    const double* X = c.dX + (size_t)q0 * c.lddx;
-    const double* B = c.dB + (size_t)q0 * c.lddb;
-    double* R = c.dR + (size_t)q0 * c.ldr;
     if (c.trans == 0) {
-        dn_gemv_n_kernel<1><<<dim3(gx(n), kResChunks, nq), kRT, 0, c.st>>>(n, nq, c.dA, c.ldda, X, c.lddx, c.part);
+        dn_gemv_n_kernel<1><<<dim3(gx(n), kResChunks, nq), kRT, 0, c.st>>>(n, nq, c.dA, c.ldda, X, ldx, c.part);
     } else {
         dn_gemv_t_kernel<1><<<dim3((unsigned)ceildiv(n, 16), kResChunks, nq), kRT, 0, c.st>>>(n, nq, c.dA, c.ldda, X,
-                                                                                              c.lddx, c.part);
+                                                                                              ldx, c.part);
     }
     MGI_LAUNCH_CHECK();
-    dn_resid_reduce_kernel<<<dim3(gx(n), nq), kRT, 0, c.st>>>(n, nq, kResChunks, c.part, B, c.lddb, X, c.lddx, R,
-                                                               c.ldr, c.nrm);
+    dn_resid_reduce_kernel<<<dim3(gx(n), nq), kRT, 0, c.st>>>(n, nq, kResChunks, c.part, B, ldb, X, ldx, R, ldr,
+                                                               c.nrm);
     MGI_LAUNCH_CHECK();
     std::vector<double> h((size_t)gx(n) * nq * 2);
     MGI_HIP_CHECK(hipMemcpyAsync(h.data(), c.nrm, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.st));
@@ -485,16 +510,26 @@ static magma_int_t ir_residual(DenseIrCtx& c, int q0, int nq, std::vector<double
     return MAGMA_SUCCESS;
 }
 
-// D (+)= op(LU)^{-1} S for nq columns (S FP64, ld lds; D FP64, ld ldd)
-static magma_int_t ir_lowsolve(DenseIrCtx& c, const double* Sv, int ldsv, int nq, double* D, int ldd, int acc)
+// D (+)= op(LU)^{-1} S for nq columns (S, D FP64): S solves round S to FP32
+// and use the FP32 factors (magma_dsgetrs_gpu), D solves use the FP64 copy
+// of the same factors (magma_dgetrs_gpu on slag2d of them).
+static magma_int_t ir_trs(DenseIrCtx& c, const double* Sv, int ldsv, int nq, double* D, int ldd, int acc)
 {
     const int n = c.n;
     dim3 g(gx(n), (unsigned)nq);
-    dn_gather_d2s_kernel<<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? c.sw.perm : nullptr, Sv, ldsv, c.Ys, n);
-    MGI_LAUNCH_CHECK();
-    magma_int_t rc = dense_trsv_both<float>(c.trans, n, nq, c.S, c.lds, c.Ys, n, c.sw, c.st);
-    if (rc != MAGMA_SUCCESS) return rc;
-    dn_apply_s2d_kernel<<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? nullptr : c.sw.perm, c.Ys, n, D, ldd, acc);
+    magma_int_t rc;
+    if (!c.dtrs) {
+        dn_gather_kernel<float><<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? c.sw.perm : nullptr, Sv, ldsv, c.Ys, n);
+        MGI_LAUNCH_CHECK();
+        if ((rc = dense_trsv_both<float>(c.trans, n, nq, c.S, c.lds, c.Ys, n, c.sw, c.st)) != MAGMA_SUCCESS) return rc;
+        dn_apply_kernel<float><<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? nullptr : c.sw.perm, c.Ys, n, D, ldd, acc);
+    } else {
+        dn_gather_kernel<double><<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? c.swd.perm : nullptr, Sv, ldsv, c.Yd, n);
+        MGI_LAUNCH_CHECK();
+        if ((rc = dense_trsv_both<double>(c.trans, n, nq, c.Dl, n, c.Yd, n, c.swd, c.st)) != MAGMA_SUCCESS) return rc;
+        dn_apply_kernel<double><<<g, kRT, 0, c.st>>>(n, nq, c.trans == 0 ? nullptr : c.swd.perm, c.Yd, n, D, ldd,
+                                                     acc);
+    }
     MGI_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }
@@ -504,144 +539,225 @@ static bool col_ok(double r, double x, double cte)
     return std::isfinite(r) && std::isfinite(x) && r <= x * cte;
 }
 
-// Classical refinement, all columns together. Returns steps taken (>= 0) in
-// *steps and whether every column converged.
-static magma_int_t ir_classical(DenseIrCtx& c, int itermax, int* steps, bool* converged)
+// Krylov workspace of one FGMRES level: V n x (m+1), Z n x m, r n, and the
+// reduction buffers.
+struct GmBuf {
+    DevBuf vb, hb;
+    double* V = nullptr; double* Z = nullptr; double* r = nullptr;
+    double* part = nullptr; double* hd = nullptr; double* hacc = nullptr; double* nrm2 = nullptr;
+    int m = 0;
+    hipError_t alloc(int n, int restart)
+    {
+        m = restart;
+        hipError_t e = vb.alloc(sizeof(double) * (size_t)n * (2 * m + 2));
+        if (e != hipSuccess) return e;
+        if ((e = hb.alloc(sizeof(double) * ((size_t)kDotChunks * (m + 2) + 4 * (m + 2)))) != hipSuccess) return e;
+        V = static_cast<double*>(vb.p);
+        Z = V + (size_t)n * (m + 1);
+        r = Z + (size_t)n * m;
+        part = static_cast<double*>(hb.p);           // kDotChunks x (m+2)
+        hd = part + (size_t)kDotChunks * (m + 2);    // m+2: h of the current pass
+        hacc = hd + (m + 2);                         // m+2: h summed over the two passes
+        nrm2 = hacc + (m + 2);                       // [0] squared norm
+        return hipSuccess;
+    }
+};
+
+// One level of the FGMRES of src/dgmres_plu_gpu.cpp (Saad, Iterative
+// Methods, Alg. 9.6) for one column: A x = b from the x given, restart g.m,
+// at most maxit inner iterations. A cycle starts from the true residual
+// r = b - A x; it stops when ||r||_2 <= tol ||r0||_2 and, for an outer
+// solve, also when the refinement test ||r||_inf <= ||x||_inf cte holds
+// (MAGMA's test alone cannot be met in FP64 once ||x|| is large). The
+// preconditioner is the triangular solve, or (nested != null) an inner
+// FGMRES of 50 iterations to 1e-4 preconditioned by it (GMGM, IRGM modes).
+// *iters: iterations of this level; *inner: iterations of the nested level.
+static magma_int_t fgmres_col(DenseIrCtx& c, const double* b, double* x, GmBuf& g, int maxit, double tol, bool outer,
+                              GmBuf* nested, int* iters, long* inner, bool* converged)
+{
+    const int n = c.n, m = g.m;
+    const unsigned g1 = gx(n);
+    std::vector<double> rn, xn;
+    magma_int_t rc;
+    int total = 0;
+    double eps1 = -1.0;
+    *converged = false;
+    for (;;) {
+        // r = b - A x, beta = ||r||_2, v_0 = r / beta
+        if ((rc = ir_residual(c, b, n, x, n, g.r, n, 1, rn, xn)) != MAGMA_SUCCESS) return rc;
+        dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, 0, g.V, g.r, g.part, 1);
+        dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, 0, 1, g.part, g.nrm2, nullptr);
+        dn_scale_kernel<<<g1, kRT, 0, c.st>>>(n, g.r, g.nrm2, g.V);
+        MGI_LAUNCH_CHECK();
+        double beta2 = 0.0;
+        MGI_HIP_CHECK(hipMemcpyAsync(&beta2, g.nrm2, sizeof(double), hipMemcpyDeviceToHost, c.st));
+        MGI_HIP_CHECK(hipStreamSynchronize(c.st));
+        const double beta = std::sqrt(beta2);
+        if (!std::isfinite(beta)) break;
+        if (eps1 < 0.0) eps1 = tol * beta;
+        if (beta <= eps1 || beta == 0.0 || (outer && col_ok(rn[0], xn[0], c.cte))) {
+            *converged = true;
+            break;
+        }
+        if (total >= maxit) break;
+        // inner stop on the residual estimate; for an outer solve also at half
+        // the refinement bound with ||x|| of the cycle start (the next cycle
+        // checks the true residual)
+        const double eps2 = outer ? std::max(eps1, 0.5 * c.cte * xn[0]) : eps1;
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0);
+        gv[0] = beta;
+        int k = 0;
+        for (int i = 0; i < m && total < maxit; ++i) {
+            double* vi = g.V + (size_t)i * n;
+            double* zi = g.Z + (size_t)i * n;
+            double* w = g.V + (size_t)(i + 1) * n;
+            if (nested) {
+                dn_zero_d_kernel<<<dim3(g1, 1), kRT, 0, c.st>>>(n, 1, zi, n);
+                MGI_LAUNCH_CHECK();
+                int it_in = 0;
+                bool conv_in = false;
+                if ((rc = fgmres_col(c, vi, zi, *nested, nested->m, 1e-4, false, nullptr, &it_in, nullptr,
+                                     &conv_in)) != MAGMA_SUCCESS)
+                    return rc;
+                if (inner) *inner += it_in;
+            } else if ((rc = ir_trs(c, vi, n, 1, zi, n, 0)) != MAGMA_SUCCESS) {
+                return rc;
+            }
+            // w = A z_i
+            if (c.trans == 0)
+                dn_gemv_n_kernel<1><<<dim3(g1, kResChunks, 1), kRT, 0, c.st>>>(n, 1, c.dA, c.ldda, zi, n, c.part);
+            else
+                dn_gemv_t_kernel<1><<<dim3((unsigned)ceildiv(n, 16), kResChunks, 1), kRT, 0, c.st>>>(
+                    n, 1, c.dA, c.ldda, zi, n, c.part);
+            MGI_LAUNCH_CHECK();
+            dn_sum_parts_kernel<<<g1, kRT, 0, c.st>>>(n, kResChunks, c.part, w);
+            MGI_LAUNCH_CHECK();
+            // two classical Gram-Schmidt passes against v_0..v_i, then ||w||
+            dn_zero_d_kernel<<<dim3(1, 1), kRT, 0, c.st>>>(i + 2, 1, g.hacc, m + 2);
+            for (int pass = 0; pass < 2; ++pass) {
+                dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, i + 1, g.V, w, g.part, 0);
+                dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, i + 1, i + 1, g.part, g.hd, g.hacc);
+                dn_orth_update_kernel<<<g1, kRT, 0, c.st>>>(n, i + 1, g.V, g.hd, w);
+            }
+            dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, 0, g.V, w, g.part, 1);
+            dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, 0, 1, g.part, g.nrm2, nullptr);
+            dn_scale_kernel<<<g1, kRT, 0, c.st>>>(n, w, g.nrm2, w);
+            MGI_LAUNCH_CHECK();
+            std::vector<double> hcol(i + 2);
+            MGI_HIP_CHECK(hipMemcpyAsync(hcol.data(), g.hacc, sizeof(double) * (i + 1), hipMemcpyDeviceToHost, c.st));
+            MGI_HIP_CHECK(hipMemcpyAsync(&beta2, g.nrm2, sizeof(double), hipMemcpyDeviceToHost, c.st));
+            MGI_HIP_CHECK(hipStreamSynchronize(c.st));
+            hcol[i + 1] = std::sqrt(beta2);
+            ++total;
+            // Givens rotations (Saad, Iterative Methods, Alg. 6.9)
+            for (int j = 0; j < i; ++j) {
+                const double a = hcol[j], bb = hcol[j + 1];
+                hcol[j] = cs[j] * a + sn[j] * bb;
+                hcol[j + 1] = -sn[j] * a + cs[j] * bb;
+            }
+            const double a = hcol[i], bb = hcol[i + 1];
+            const double den = std::hypot(a, bb);
+            cs[i] = den > 0.0 ? a / den : 1.0;
+            sn[i] = den > 0.0 ? bb / den : 0.0;
+            hcol[i] = den;
+            hcol[i + 1] = 0.0;
+            gv[i + 1] = -sn[i] * gv[i];
+            gv[i] = cs[i] * gv[i];
+            for (int j = 0; j <= i; ++j) H[(size_t)j + (size_t)i * (m + 1)] = hcol[j];
+            k = i + 1;
+            if (!std::isfinite(gv[i + 1])) break;
+            if (std::fabs(gv[i + 1]) <= eps2) break;
+        }
+        // y = H(0:k,0:k)^{-1} g, x += Z y
+        std::vector<double> y(k, 0.0);
+        for (int j = k - 1; j >= 0; --j) {
+            double sum = gv[j];
+            for (int l = j + 1; l < k; ++l) sum -= H[(size_t)j + (size_t)l * (m + 1)] * y[l];
+            const double d = H[(size_t)j + (size_t)j * (m + 1)];
+            y[j] = d != 0.0 ? sum / d : 0.0;
+        }
+        if (k > 0) {
+            MGI_HIP_CHECK(hipMemcpyAsync(g.hd, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.st));
+            dn_xupdate_kernel<<<g1, kRT, 0, c.st>>>(n, k, g.Z, g.hd, x);
+            MGI_LAUNCH_CHECK();
+            MGI_HIP_CHECK(hipStreamSynchronize(c.st));   // y is pageable host memory
+        }
+        // an inner solve is one cycle (restart = maxit = 50), ended on the
+        // residual estimate as in src/dgmres_plu_gpu.cpp
+        if (!outer) {
+            *converged = k > 0 && std::fabs(gv[k]) <= eps2;
+            break;
+        }
+    }
+    *iters = total;
+    return MAGMA_SUCCESS;
+}
+
+// Classical refinement (IRSTRS, IRDTRS, IRGMSTRS, IRGMDTRS), all columns
+// together: x = LU^{-1} b, then x += d with d = LU^{-1} r, or d from an inner
+// FGMRES on A d = r (nested != null), until every column passes the test.
+// *steps: outer steps, or with an inner solver its iterations in total.
+static magma_int_t ir_classical(DenseIrCtx& c, int itermax, GmBuf* nested, double* dz, int* steps, bool* converged)
 {
     const int n = c.n, nrhs = c.nrhs;
     *converged = false;
     *steps = 0;
-    // x = LU^{-1} b
-    magma_int_t rc = ir_lowsolve(c, c.dB, c.lddb, nrhs, c.dX, c.lddx, 0);
+    long tot_inner = 0;
+    magma_int_t rc = ir_trs(c, c.dB, c.lddb, nrhs, c.dX, c.lddx, 0);
     if (rc != MAGMA_SUCCESS) return rc;
     std::vector<double> rn, xn;
     for (int it = 0; it <= itermax; ++it) {
-        if ((rc = ir_residual(c, 0, nrhs, rn, xn)) != MAGMA_SUCCESS) return rc;
+        if ((rc = ir_residual(c, c.dB, c.lddb, c.dX, c.lddx, c.dR, c.ldr, nrhs, rn, xn)) != MAGMA_SUCCESS) return rc;
         bool all = true, bad = false;
         for (int q = 0; q < nrhs; ++q) {
             if (!std::isfinite(rn[q]) || !std::isfinite(xn[q])) bad = true;
             if (!col_ok(rn[q], xn[q], c.cte)) all = false;
         }
-        *steps = it;
+        *steps = nested ? (int)std::min<long>(tot_inner, INT32_MAX) : it;
         if (all) { *converged = true; return MAGMA_SUCCESS; }
         if (bad || it == itermax) return MAGMA_SUCCESS;
-        if ((rc = ir_lowsolve(c, c.dR, c.ldr, nrhs, c.dX, c.lddx, 1)) != MAGMA_SUCCESS) return rc;
+        if (!nested) {
+            if ((rc = ir_trs(c, c.dR, c.ldr, nrhs, c.dX, c.lddx, 1)) != MAGMA_SUCCESS) return rc;
+            continue;
+        }
+        for (int q = 0; q < nrhs; ++q) {
+            dn_zero_d_kernel<<<dim3(gx(n), 1), kRT, 0, c.st>>>(n, 1, dz, n);
+            MGI_LAUNCH_CHECK();
+            int it_in = 0;
+            bool conv_in = false;
+            if ((rc = fgmres_col(c, c.dR + (size_t)q * c.ldr, dz, *nested, nested->m, 1e-4, false, nullptr, &it_in,
+                                 nullptr, &conv_in)) != MAGMA_SUCCESS)
+                return rc;
+            tot_inner += it_in;
+            dn_apply_kernel<double><<<dim3(gx(n), 1), kRT, 0, c.st>>>(n, 1, nullptr, dz, n,
+                                                                      c.dX + (size_t)q * c.lddx, c.lddx, 1);
+            MGI_LAUNCH_CHECK();
+        }
     }
-    (void)n;
     return MAGMA_SUCCESS;
 }
 
-// FGMRES per column (x starts at 0). *steps = total inner iterations.
-static magma_int_t ir_fgmres(DenseIrCtx& c, int itermax, int* steps, bool* converged)
+// FGMRES refinement (GMSTRS, GMDTRS, GMGMSTRS, GMGMDTRS) per column from
+// x = 0. *steps: outer iterations, or with an inner solver its iterations.
+static magma_int_t ir_fgmres(DenseIrCtx& c, int itermax, GmBuf& outer, GmBuf* nested, int* steps, bool* converged)
 {
-    const int n = c.n, m = kGmresRestart;
     *steps = 0;
     *converged = true;
-    DevBuf vb, hb;
-    MGI_HIP_CHECK(vb.alloc(sizeof(double) * (size_t)n * (2 * m + 1)));
-    MGI_HIP_CHECK(hb.alloc(sizeof(double) * ((size_t)kDotChunks * (m + 2) + 4 * (m + 2))));
-    double* V = static_cast<double*>(vb.p);           // n x (m+1)
-    double* Z = V + (size_t)n * (m + 1);              // n x m
-    double* part = static_cast<double*>(hb.p);        // kDotChunks x (m+2)
-    double* hd = part + (size_t)kDotChunks * (m + 2); // m+2: h of the current pass
-    double* hacc = hd + (m + 2);                      // m+2: h summed over the two passes
-    double* nrm2 = hacc + (m + 2);                    // [0] squared norm
-    const unsigned g1 = gx(n);
-    std::vector<double> rn, xn;
-    magma_int_t rc;
     int total = 0;
+    long tot_inner = 0;
     for (int col = 0; col < c.nrhs; ++col) {
         double* x = c.dX + (size_t)col * c.lddx;
-        dn_zero_d_kernel<<<dim3(g1, 1), kRT, 0, c.st>>>(n, 1, x, c.lddx);
+        double* b = c.dB + (size_t)col * c.lddb;
+        dn_zero_d_kernel<<<dim3(gx(c.n), 1), kRT, 0, c.st>>>(c.n, 1, x, c.lddx);
         MGI_LAUNCH_CHECK();
-        double eps1 = -1.0;
-        for (;;) {
-            // r = b - A x, beta = ||r||_2 (a cycle also starts with the true residual)
-            if ((rc = ir_residual(c, col, 1, rn, xn)) != MAGMA_SUCCESS) return rc;
-            double* r = c.dR + (size_t)col * c.ldr;
-            dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, 0, V, r, part, 1);
-            dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, 0, 1, part, nrm2, nullptr);
-            dn_scale_kernel<<<g1, kRT, 0, c.st>>>(n, r, nrm2, V);
-            MGI_LAUNCH_CHECK();
-            double beta2 = 0.0;
-            MGI_HIP_CHECK(hipMemcpyAsync(&beta2, nrm2, sizeof(double), hipMemcpyDeviceToHost, c.st));
-            MGI_HIP_CHECK(hipStreamSynchronize(c.st));
-            const double beta = std::sqrt(beta2);
-            if (!std::isfinite(beta)) { *converged = false; *steps = total; return MAGMA_SUCCESS; }
-            if (eps1 < 0.0) eps1 = c.cte * beta;
-            if (beta <= eps1 || beta == 0.0) break;
-            if (total >= itermax) { *converged = false; break; }
-            std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0);
-            gv[0] = beta;
-            int k = 0;
-            for (int i = 0; i < m && total < itermax; ++i) {
-                double* vi = V + (size_t)i * n;
-                double* zi = Z + (size_t)i * n;
-                double* w = V + (size_t)(i + 1) * n;
-                if ((rc = ir_lowsolve(c, vi, n, 1, zi, n, 0)) != MAGMA_SUCCESS) return rc;
-                // w = A z_i
-                if (c.trans == 0)
-                    dn_gemv_n_kernel<1><<<dim3(g1, kResChunks, 1), kRT, 0, c.st>>>(n, 1, c.dA, c.ldda, zi, n, c.part);
-                else
-                    dn_gemv_t_kernel<1><<<dim3((unsigned)ceildiv(n, 16), kResChunks, 1), kRT, 0, c.st>>>(
-                        n, 1, c.dA, c.ldda, zi, n, c.part);
-                MGI_LAUNCH_CHECK();
-                dn_sum_parts_kernel<<<g1, kRT, 0, c.st>>>(n, kResChunks, c.part, w);
-                MGI_LAUNCH_CHECK();
-                // two classical Gram-Schmidt passes against v_0..v_i, then ||w||
-                dn_zero_d_kernel<<<dim3(1, 1), kRT, 0, c.st>>>(i + 2, 1, hacc, m + 2);
-                for (int pass = 0; pass < 2; ++pass) {
-                    dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, i + 1, V, w, part, 0);
-                    dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, i + 1, i + 1, part, hd, hacc);
-                    dn_orth_update_kernel<<<g1, kRT, 0, c.st>>>(n, i + 1, V, hd, w);
-                }
-                dn_dots_kernel<<<kDotChunks, kRT, 0, c.st>>>(n, 0, V, w, part, 1);
-                dn_dots_reduce_kernel<<<1, 64, 0, c.st>>>(kDotChunks, 0, 1, part, nrm2, nullptr);
-                dn_scale_kernel<<<g1, kRT, 0, c.st>>>(n, w, nrm2, w);
-                MGI_LAUNCH_CHECK();
-                std::vector<double> hcol(i + 2);
-                MGI_HIP_CHECK(hipMemcpyAsync(hcol.data(), hacc, sizeof(double) * (i + 1), hipMemcpyDeviceToHost,
-                                             c.st));
-                MGI_HIP_CHECK(hipMemcpyAsync(&beta2, nrm2, sizeof(double), hipMemcpyDeviceToHost, c.st));
-                MGI_HIP_CHECK(hipStreamSynchronize(c.st));
-                hcol[i + 1] = std::sqrt(beta2);
-                ++total;
-                // Givens rotations (Saad, Iterative Methods, Alg. 6.9)
-                for (int j = 0; j < i; ++j) {
-                    const double a = hcol[j], b = hcol[j + 1];
-                    hcol[j] = cs[j] * a + sn[j] * b;
-                    hcol[j + 1] = -sn[j] * a + cs[j] * b;
-                }
-                const double a = hcol[i], b = hcol[i + 1];
-                const double den = std::hypot(a, b);
-                cs[i] = den > 0.0 ? a / den : 1.0;
-                sn[i] = den > 0.0 ? b / den : 0.0;
-                hcol[i] = den;
-                hcol[i + 1] = 0.0;
-                gv[i + 1] = -sn[i] * gv[i];
-                gv[i] = cs[i] * gv[i];
-                for (int j = 0; j <= i; ++j) H[(size_t)j + (size_t)i * (m + 1)] = hcol[j];
-                k = i + 1;
-                if (!std::isfinite(gv[i + 1])) break;
-                if (std::fabs(gv[i + 1]) <= eps1) break;
-            }
-            // y = H(0:k,0:k)^{-1} g, x += Z y
-            std::vector<double> y(k, 0.0);
-            for (int j = k - 1; j >= 0; --j) {
-                double s = gv[j];
-                for (int l = j + 1; l < k; ++l) s -= H[(size_t)j + (size_t)l * (m + 1)] * y[l];
-                const double d = H[(size_t)j + (size_t)j * (m + 1)];
-                y[j] = d != 0.0 ? s / d : 0.0;
-            }
-            if (k > 0) {
-                MGI_HIP_CHECK(hipMemcpyAsync(hd, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.st));
-                dn_xupdate_kernel<<<g1, kRT, 0, c.st>>>(n, k, Z, hd, x);
-                MGI_LAUNCH_CHECK();
-                MGI_HIP_CHECK(hipStreamSynchronize(c.st));   // y is pageable host memory
-            }
-        }
+        int it = 0;
+        bool conv = false;
+        magma_int_t rc = fgmres_col(c, b, x, outer, itermax, c.cte, true, nested, &it, &tot_inner, &conv);
+        if (rc != MAGMA_SUCCESS) return rc;
+        total += it;
+        if (!conv) { *converged = false; break; }
     }
-    *steps = total;
+    *steps = nested ? (int)std::min<long>(tot_inner, INT32_MAX) : total;
     return MAGMA_SUCCESS;
 }
 
@@ -657,33 +773,35 @@ static void swp2pswp(int trans, int n, const int* ipiv, std::vector<int>& out)
 }
 
 magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* ipiv, int* dipiv, double* dB,
-                         int lddb, double* dX, int lddx, double* dworkd, float* dworks, int mode, bool gmres,
+                         int lddb, double* dX, int lddx, double* dworkd, float* dworks, int mode, int solver,
                          int* iter, int* info)
 {
     *iter = 0;
     *info = 0;
     const int gmax = std::max(dense_panel_gmax(n), 1);
     if (gmax > device_cu_count()) return MAGMA_ERR_NOT_SUPPORTED;
+    const bool gm_outer = (solver & kRefineOuterGM) != 0;
+    const bool nested = (solver & kRefineInnerGM) != 0;
     StreamHolder sh;
     MGI_HIP_CHECK(hipStreamCreateWithFlags(&sh.s, hipStreamNonBlocking));
     hipStream_t st = sh.s;
     const int nchunk = (int)ceildiv(n, 256);
     const int nrhs_max = std::max(nrhs, 1);
-    // scratch: FP32 LU (ld n), FP32 staging, LU work, solve work, partial sums
     // the caller's FP32 workspace (n*(n+nrhs) floats) holds the FP32 factors
     // (ld n) and the FP32 right-hand sides, as MAGMA documents dworks
     const size_t lu_bytes = dworks ? 0 : a256_bytes(sizeof(float) * (size_t)n * n);
     DevBuf big;
     const size_t ys_bytes = dworks ? 0 : a256_bytes(sizeof(float) * (size_t)n * nrhs_max);
-    const size_t dw_bytes = a256_bytes(dense_work_bytes(n, 4, gmax));
+    const size_t dw_bytes = a256_bytes(dense_work_bytes(n, n, 4, gmax));
     const size_t sw_bytes = a256_bytes(solve_work_bytes(n, std::min(nrhs_max, 64), 4));
     const size_t part_elems = std::max<size_t>((size_t)kResChunks * nrhs_max * n, (size_t)nchunk * n);
     const size_t part_bytes = a256_bytes(sizeof(double) * part_elems);
     const size_t nrm_bytes = a256_bytes(sizeof(double) * ((size_t)gx(n) * nrhs_max * 2 + 64));
     const size_t r_bytes = dworkd ? 0 : a256_bytes(sizeof(double) * (size_t)n * nrhs_max);
     const size_t small_bytes = a256_bytes(sizeof(int) * ((size_t)n + 64));
+    const size_t dz_bytes = a256_bytes(sizeof(double) * (size_t)n);
     MGI_HIP_CHECK(big.alloc(lu_bytes + ys_bytes + dw_bytes + sw_bytes + part_bytes + nrm_bytes + r_bytes +
-                            small_bytes));
+                            small_bytes + dz_bytes));
     char* p = static_cast<char*>(big.p);
     DenseIrCtx c{};
     c.trans = trans; c.n = n; c.nrhs = nrhs;
@@ -696,15 +814,17 @@ magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* 
         c.Ys = reinterpret_cast<float*>(p);                   p += ys_bytes;
     }
     DenseWork dw;
-    dense_work_carve(p, n, 4, gmax, dw);                      p += dw_bytes;
+    dense_work_carve(p, n, n, 4, gmax, dw);                      p += dw_bytes;
     solve_work_carve(p, n, std::min(nrhs_max, 64), 4, c.sw);  p += sw_bytes;
     c.part = reinterpret_cast<double*>(p);                    p += part_bytes;
     c.nrm = reinterpret_cast<double*>(p);                     p += nrm_bytes;
     if (dworkd) { c.dR = dworkd; c.ldr = n; }
     else { c.dR = reinterpret_cast<double*>(p); c.ldr = n;    p += r_bytes; }
-    int* d_small = reinterpret_cast<int*>(p);                 // [0] ovf, [1] info, [2] bad, [64..] ipiv
+    int* d_small = reinterpret_cast<int*>(p);                 p += small_bytes;  // [0] ovf, [1] info, [2] bad, [64..] ipiv
     int* d_lpipiv = d_small + 64;
+    double* dz = reinterpret_cast<double*>(p);
     c.st = st;
+    c.dtrs = (solver & kRefineDTRS) != 0;
 
     dn_zero_i_kernel<<<1, 64, 0, st>>>(d_small, 64);
     // 1. down-cast with overflow check, ||A||_inf
@@ -723,6 +843,7 @@ magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* 
     c.cte = anrm * eps * std::sqrt((double)n);
     int it = 0;
     bool fallback = false;
+    DevBuf dcopy, dsw;
     if (h_small[0] != 0) {
         it = -2;
         fallback = true;
@@ -746,15 +867,37 @@ magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* 
                 MGI_HIP_CHECK(hipMemcpyAsync(dipiv, rowmap.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
                 MGI_HIP_CHECK(hipStreamSynchronize(st));
             }
-            if ((rc = dense_set_perm(n, ipiv, c.sw, st)) != MAGMA_SUCCESS) return rc;
-            if ((rc = dense_diag_inverses<float>(n, c.S, c.lds, c.sw, st)) != MAGMA_SUCCESS) return rc;
-            // 3. refinement
-            const int itermax = std::min(n, 500);
+            if (c.dtrs) {
+                // FP64 copy of the factors (slag2d) with its own solve state
+                const int nr = std::min(nrhs_max, 64);
+                MGI_HIP_CHECK(dcopy.alloc(sizeof(double) * ((size_t)n * n + (size_t)n * nrhs_max)));
+                MGI_HIP_CHECK(dsw.alloc(solve_work_bytes(n, nr, 8)));
+                c.Dl = static_cast<double*>(dcopy.p);
+                c.Yd = c.Dl + (size_t)n * n;
+                solve_work_carve(dsw.p, n, nr, 8, c.swd);
+                dn_lag2d_kernel<<<dim3(gx(n), (unsigned)n), kRT, 0, st>>>(n, c.S, c.lds, c.Dl);
+                MGI_LAUNCH_CHECK();
+                if ((rc = dense_set_perm(n, ipiv, c.swd, st)) != MAGMA_SUCCESS) return rc;
+                if ((rc = dense_diag_inverses<double>(n, c.Dl, n, c.swd, st)) != MAGMA_SUCCESS) return rc;
+            } else {
+                if ((rc = dense_set_perm(n, ipiv, c.sw, st)) != MAGMA_SUCCESS) return rc;
+                if ((rc = dense_diag_inverses<float>(n, c.S, c.lds, c.sw, st)) != MAGMA_SUCCESS) return rc;
+            }
+            // 3. refinement; ITERMAX = min(n, 500), an outer level over an
+            // inner FGMRES runs max(10, ITERMAX / 50) steps, and the outer
+            // FGMRES restarts after as many iterations as it may take
+            // (src/dxgesv_gmres_gpu.cpp:211-219, 328-359, 574-575)
+            const int itermax_all = std::min(n, 500);
+            const int itermax = nested ? std::max(10, itermax_all / 50) : itermax_all;
+            GmBuf gin, gout;
+            if (nested) MGI_HIP_CHECK(gin.alloc(n, std::min(n, kGmresInner)));
+            if (gm_outer) MGI_HIP_CHECK(gout.alloc(n, itermax));
             int steps = 0;
             bool conv = false;
-            rc = gmres ? ir_fgmres(c, itermax, &steps, &conv) : ir_classical(c, itermax, &steps, &conv);
+            rc = gm_outer ? ir_fgmres(c, itermax, gout, nested ? &gin : nullptr, &steps, &conv)
+                          : ir_classical(c, itermax, nested ? &gin : nullptr, dz, &steps, &conv);
             if (rc != MAGMA_SUCCESS) return rc;
-            if ((rc = dense_solve_check_err(c.sw, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = dense_solve_check_err(c.dtrs ? c.swd : c.sw, st)) != MAGMA_SUCCESS) return rc;
             if (conv) it = steps;
             else { it = -std::max(1, steps); fallback = true; }
         }

@@ -8,11 +8,15 @@
 
 namespace mgi {
 
-// Panel width of the cooperative panel kernel and the block width of the
-// blocked LU (two panels per block): FP64 64 / 128, FP32 128 / 256.
+// Panel width of the cooperative panel kernel and the default block width of
+// the blocked LU (a multiple of the panel width): FP64 64 / 256, FP32 128 / 512.
 template <typename T> struct DenseShape;
-template <> struct DenseShape<double> { static constexpr int PW = 64;  static constexpr int NB = 128; };
-template <> struct DenseShape<float>  { static constexpr int PW = 128; static constexpr int NB = 256; };
+template <> struct DenseShape<double> { static constexpr int PW = 64;  static constexpr int NB = 256; };
+template <> struct DenseShape<float>  { static constexpr int PW = 128; static constexpr int NB = 512; };
+// Largest block width (row pairs of one block), and the strip height of the
+// triangular solves (order of the diagonal-block inverses).
+constexpr int kDenseMaxNB = 512;
+constexpr int kDenseStrip = 128;
 
 // Rows held by one workgroup of the panel kernel (one row per thread).
 constexpr int kDensePanelRows = 256;
@@ -21,19 +25,20 @@ constexpr int kSolveBS = 128;
 
 // Device scratch of the dense LU, carved from one buffer by dense_work_bytes.
 struct DenseWork {
-    int* tag;        // [2][gmax] step tag of each workgroup's published candidate
-    int* cpos;       // [2][gmax] LAPACK position of the candidate (-1: none)
-    void* crow;      // [2][gmax][PW] candidate rows (T)
+    void* crow;      // [2][gmax][1 + PW * sizeof(T)/4] tagged 64-bit words of the candidates
+    int nwords;      // number of those words
     int* err;        // [0] a wait in the panel kernel exceeded its time limit
-    int* pairs;      // [2][2*NB] (source row, destination row) of a block's row moves
+    int* pairs;      // [2][2*kDenseMaxNB] (source row, destination row) of a block's row moves
     int* npairs;     // [0] number of pairs
-    void* X;         // NB x NB inverse of a diagonal block (T)
-    void* W;         // NB x n: U12 before it is copied into A (T)
+    void* X;         // kDenseStrip x kDenseStrip inverse of a diagonal block (T)
+    void* Ah;        // FP32 storage: 16-bit A21 (rows, k contiguous), hi then lo halves of h_elems each
+    void* Bh;        // FP32 storage: 16-bit U12 (columns, k contiguous), likewise
+    size_t h_elems;  // capacity of each half
     int gmax;        // panel workgroups the buffers hold
 };
 
-size_t dense_work_bytes(int n, int elem, int gmax);
-void dense_work_carve(void* base, int n, int elem, int gmax, DenseWork& w);
+size_t dense_work_bytes(int m, int n, int elem, int gmax);
+void dense_work_carve(void* base, int m, int n, int elem, int gmax, DenseWork& w);
 int dense_panel_gmax(int m);
 int device_cu_count();
 
@@ -44,10 +49,12 @@ int device_cu_count();
 // (kUpdF32, kUpdBF16, kUpdFP16, kUpdBF16x3). The triangular solves for U12
 // run in the storage precision. Returns MAGMA_ERR_NOT_SUPPORTED when the
 // matrix has more rows than the panel kernel can hold on the device.
+// nb: block width (a multiple of the panel width, at most kDenseMaxNB), 0 for
+// the default.
 magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipiv, int* d_info,
-                        const DenseWork& w, hipStream_t stream);
+                        const DenseWork& w, hipStream_t stream, int nb = 0);
 magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info,
-                         const DenseWork& w, hipStream_t stream);
+                         const DenseWork& w, hipStream_t stream, int nb = 0);
 // Reads the panel kernel's error word (synchronises the stream).
 magma_int_t dense_check_err(const DenseWork& w, hipStream_t stream);
 
@@ -85,14 +92,21 @@ magma_int_t dense_trsv_both(int trans, int n, int nrhs, const T* LU, int ld, T* 
                             hipStream_t st);
 magma_int_t dense_solve_check_err(const SolveWork& s, hipStream_t stream);
 
+// Refinement solver bits: solver = magma_refinement_t - Magma_REFINE_IRSTRS
+// gives IRSTRS 0, IRDTRS 1, IRGMSTRS 2, IRGMDTRS 3, GMSTRS 4, GMDTRS 5,
+// GMGMSTRS 6, GMGMDTRS 7 (include/magma_types.h order).
+constexpr int kRefineDTRS = 1;      // correction solves on the FP64 copy of the factors
+constexpr int kRefineInnerGM = 2;   // preconditioner: inner FGMRES over the triangular solves
+constexpr int kRefineOuterGM = 4;   // outer FGMRES (else classical refinement)
+
 // Refinement solve of one system (dense_drivers.hip): mode is the update
-// format of the FP32 LU, gmres selects FGMRES (else classical refinement).
-// ipiv (host, n) receives the pivots used, dipiv (device, nullable) their
-// row map. dworkd (nullable, n*nrhs doubles) holds the residual and dworks
-// (nullable, n*(n+nrhs) floats) the FP32 factors. *iter follows MAGMA's
-// convention; on fallback dA holds its FP64 LU.
+// format of the FP32 LU, solver the bits above. ipiv (host, n) receives the
+// pivots used, dipiv (device, nullable) their row map. dworkd (nullable,
+// n*nrhs doubles) holds the residual and dworks (nullable, n*(n+nrhs)
+// floats) the FP32 factors. *iter follows MAGMA's convention; on fallback
+// dA holds its FP64 LU.
 magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* ipiv, int* dipiv, double* dB,
-                         int lddb, double* dX, int lddx, double* dworkd, float* dworks, int mode, bool gmres,
+                         int lddb, double* dX, int lddx, double* dworkd, float* dworks, int mode, int solver,
                          int* iter, int* info);
 
 }  // namespace mgi

@@ -15,8 +15,9 @@ namespace {
 
 // Panel workgroups the scratch holds: m <= 256 * 256 = 65,536 rows.
 constexpr int kNaGmax = 256;
-constexpr size_t kNaBytes = 512 * 1024;
+constexpr size_t kNaBytes = 1024 * 1024;
 
+// 1 MB: the panel's slot words (528 KB), one block inverse (128 KB), pairs.
 // Every device holds its own copy until the process ends. Calls that overlap
 // on one device share it and race.
 __device__ __align__(256) unsigned char scratch[kNaBytes];
@@ -76,12 +77,12 @@ magma_int_t magma_dgetrf_native_noalloc(magma_int_t m, magma_int_t n, magmaDoubl
                      256 * kNaGmax);
         return MAGMA_ERR_NOT_SUPPORTED;
     }
-    if (dense_work_bytes(n, (int)sizeof(double), kNaGmax) > kNaBytes) return MAGMA_ERR_NOT_SUPPORTED;
+    if (dense_work_bytes(m, n, (int)sizeof(double), kNaGmax) > kNaBytes) return MAGMA_ERR_NOT_SUPPORTED;
     MGI_HIP_CHECK(hipSetDevice(queue->device));
     void* base = scratch_on(queue->device);
     if (base == nullptr) return MAGMA_ERR_UNKNOWN;
     DenseWork w;
-    dense_work_carve(base, n, (int)sizeof(double), kNaGmax, w);
+    dense_work_carve(base, m, n, (int)sizeof(double), kNaGmax, w);
     magma_int_t rc = dense_getrf(kUpdF64, m, n, dA, ldda, d_ipiv, d_info, w, queue->stream);
     if (rc != MAGMA_SUCCESS) return rc;
     na_err_to_info_kernel<<<1, 1, 0, queue->stream>>>(w.err, d_info);

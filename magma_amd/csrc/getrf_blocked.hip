@@ -72,7 +72,10 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
     const int t = threadIdx.x;
     const int w = t / kWave;
     T* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
-    int* ipiv = ipiv_array[b];
+    // a null pivot array asks for the factorisation without pivoting
+    // (src/zgetrf_nopiv_batched.cpp): the diagonal entry is the pivot
+    const bool nopiv = ipiv_array == nullptr;
+    int* ipiv = nopiv ? nullptr : ipiv_array[b];
 
     T a[R][NB];
 #pragma unroll
@@ -95,11 +98,11 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 const int r = t + kPanelThreads * i;
-                if (r >= j && r < mloc) {
+                if (r >= j && r < mloc && (!nopiv || r == j)) {
                     // idamax: a NaN never wins a comparison, except in the
                     // first candidate, which keeps the pivot (counted as +inf)
                     T v = fabs(a[i][j]);
-                    if (r == j && v != v) v = T(INFINITY);
+                    if (r == j && (v != v || nopiv)) v = T(INFINITY);
                     if (v > best) { best = v; bi = r; bs = i; }
                 }
             }
@@ -137,7 +140,7 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
             const T* prow = s_cand[par][pw];
             const T* jrow = s_rowj[par];
             const T pv = prow[j];
-            if (t == 0) ipiv[k + j] = k + p + 1;
+            if (t == 0 && !nopiv) ipiv[k + j] = k + p + 1;
             T rinv = T(1);
             bool recip = true;
             if (pv != T(0)) {
@@ -570,9 +573,11 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
         magma_int_t prc = panel_launch<T, NB>(R, sz, k, dA_array, ipiv_array, info_array, batch, stream);
         if (prc != MAGMA_SUCCESS) return prc;
         const int other = max_n;   // upper bound of the columns outside the panel
-        laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
-                                                                                   ipiv_array, 0);
-        MGI_LAUNCH_CHECK();
+        if (ipiv_array) {
+            laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
+                                                                                       ipiv_array, 0);
+            MGI_LAUNCH_CHECK();
+        }
         const int rest_n = max_n - k - 1;
         const int rest_m = max_m - k - 1;
         if (rest_n > 0) {
@@ -604,8 +609,10 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
         // block's second half only
         magma_int_t prc = panel_launch<T, H>(R, sz, k, dA_array, ipiv_array, info_array, batch, stream);
         if (prc != MAGMA_SUCCESS) return prc;
-        laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k, H, dA_array, ipiv_array, kInPanelRight);
-        MGI_LAUNCH_CHECK();
+        if (ipiv_array) {
+            laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k, H, dA_array, ipiv_array, kInPanelRight);
+            MGI_LAUNCH_CHECK();
+        }
         trsm_kernel<T, H><<<dim3(1, batch), 256, 0, stream>>>(sz, k, dA_array, kInPanelRight);
         MGI_LAUNCH_CHECK();
         if (max_m - k - H > 0) {
@@ -617,13 +624,18 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
         if (k + H < max_minmn) {
             prc = panel_launch<T, H>(R, sz, k + H, dA_array, ipiv_array, info_array, batch, stream);
             if (prc != MAGMA_SUCCESS) return prc;
-            laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k + H, H, dA_array, ipiv_array, kInPanelLeft);
-            MGI_LAUNCH_CHECK();
+            if (ipiv_array) {
+                laswp_kernel<T><<<dim3(1, batch), 256, 0, stream>>>(sz, k + H, H, dA_array, ipiv_array,
+                                                                    kInPanelLeft);
+                MGI_LAUNCH_CHECK();
+            }
         }
         // the whole block's interchanges outside it, U12 and the trailing update
-        laswp_kernel<T><<<dim3(ceildiv(max_n, kLaswpCols), batch), 256, 0, stream>>>(sz, k, 2 * H, dA_array,
-                                                                                   ipiv_array, 0);
-        MGI_LAUNCH_CHECK();
+        if (ipiv_array) {
+            laswp_kernel<T><<<dim3(ceildiv(max_n, kLaswpCols), batch), 256, 0, stream>>>(sz, k, 2 * H, dA_array,
+                                                                                       ipiv_array, 0);
+            MGI_LAUNCH_CHECK();
+        }
         const int rest_n = max_n - k - 1;
         const int rest_m = max_m - k - 1;
         if (rest_n > 0) {

@@ -3,7 +3,8 @@
 // src/zgetrf_gpu.cpp:213-452 and :542-663, and the FP32 LU with low-precision
 // trailing updates of src/xshgetrf_gpu.cpp:86-419).
 //
-// Blocks of NB columns, each factored as two panels of PW columns:
+// Blocks of nb columns (FP64 256, FP32 512), each factored as panels of PW
+// columns (FP64 64, FP32 128):
 //   panel    dense_panel_kernel: the (m-k) x PW panel is spread over G
 //            workgroups, one row per thread in registers. Per column every
 //            workgroup publishes its best candidate row; after all G have
@@ -17,12 +18,16 @@
 //            rows in every other column, one wave per column.
 //   U12      L11^{-1} (dense_trtri_kernel) times A12 on MFMA, in the storage
 //            precision (FP64, or FP32 as MAGMA's strsm).
-//   update   A22 -= A21 U12 on MFMA (dense_gemm_kernel, 128 x 128 tiles):
-//            FP64; or FP32 storage with FP32, BF16, FP16 or split-BF16
-//            inputs and FP32 accumulation.
-// Inside a block, the first panel's interchanges, U12 and update are applied
-// to the block's second half before the second panel; the whole block's
-// interchanges, U12 and update are then applied once to the rest.
+//   update   A22 -= A21 U12 on MFMA, 128 x 128 tiles: FP64 or FP32
+//            (dense_gemm_kernel); or FP32 storage with BF16, FP16 or
+//            split-BF16 inputs and FP32 accumulation, A21 and U12 converted
+//            once per update to 16-bit rows / columns with k contiguous
+//            (dense_cvt_a/b_kernel, dense_gemm_h_kernel).
+// Inside a block, each panel's interchanges go to the block's other columns
+// and its U rows and update to the block's columns on its right before the
+// next panel; the whole block's interchanges, U12 and update (K = nb) are
+// then applied once to the rest. The triangular solves for U run in strips
+// of 128 rows (the inverse of each strip's diagonal block, then a GEMM).
 #include "magma_internal.h"
 #include "lu_device.h"
 #include "dense_kernels.h"
@@ -68,15 +73,53 @@ typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // Panel: G workgroups of 256 threads; thread t of workgroup g holds row
 // k + 256 g + t (columns k .. k+jb-1) in registers.
+//
+// Exchange between the workgroups, per column: workgroup g writes its best
+// candidate into slot (column parity, g) as 64-bit words that each carry the
+// column's tag in their upper half: word 0 = tag | position, then the row's
+// values from the current column on, 32 bits per word (FP64 values take two
+// words). A reader accepts a word only when its tag is the current column's,
+// so every word validates itself: no fence, flag or ordering between words
+// is needed, only the single-copy atomicity of aligned 64-bit accesses.
+// Wave 0 reads every slot's position and current value, picks the pivot, then
+// reads the winning row; both reads repeat until all their words carry the
+// tag (with a time limit that raises the error word instead of hanging).
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ unsigned piece_of(T v, int q)
+{
+    if constexpr (sizeof(T) == 8) return (unsigned)(q == 0 ? __double2hiint(v) : __double2loint(v));
+    else return (unsigned)__float_as_int(v);
+}
+
+template <typename T>
+__device__ __forceinline__ T value_of(unsigned hi, unsigned lo)
+{
+    if constexpr (sizeof(T) == 8) return __hiloint2double((int)hi, (int)lo);
+    else return __int_as_float((int)hi);
+}
+
+__device__ __forceinline__ unsigned long long ld_word(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_word(unsigned long long* p, unsigned tag, unsigned payload)
+{
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <typename T, int PW>
 __global__ void __launch_bounds__(kDensePanelRows)
 dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __restrict__ ipiv, int* __restrict__ info,
                    DenseWork w)
 {
     constexpr int kW = kDensePanelRows / kWave;
+    constexpr int NW = sizeof(T) / 4;              // words per value
+    constexpr int SW = 1 + PW * NW;                 // words per slot
     __shared__ T s_wv[2][kW];       // per wave: best |a| and its position (parity-buffered)
     __shared__ int s_wp[2][kW];
+    __shared__ T s_cand[2][kW][PW]; // per wave: its best row
     __shared__ T s_u[2][PW];        // pivot row
     __shared__ int s_pp[2];         // pivot position
 
@@ -91,7 +134,7 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
     bool active = valid;             // not yet chosen as a pivot
     T* P = A + k + (size_t)k * lda;
     const int gmax = w.gmax;
-    T* crow = reinterpret_cast<T*>(w.crow);
+    unsigned long long* words = reinterpret_cast<unsigned long long*>(w.crow);
 
     T a[PW];
 #pragma unroll
@@ -102,7 +145,9 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
         constexpr int par = j & 1;
         if (j < jb) {
             const int diag = k + j;
-            // 1. best candidate of each wave, then of the workgroup
+            const unsigned tag = (unsigned)diag;
+            // 1. best candidate of each wave (LAPACK idamax: largest |a|,
+            //    lowest position; a NaN wins only at the first position)
             T v = T(-1);
             int vp = INT_MAX;
             if (active) {
@@ -112,54 +157,60 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
             }
             argmax_t(v, vp);
             if (ln == 0) { s_wv[par][wv] = v; s_wp[par][wv] = vp; }
+            if (vp != INT_MAX && active && pos == vp) {
+#pragma unroll
+                for (int c = j; c < PW; ++c) s_cand[par][wv][c] = a[c];
+            }
             __syncthreads();
-            T bv = s_wv[par][0];
-            int bp = s_wp[par][0];
-#pragma unroll
-            for (int q = 1; q < kW; ++q) {
-                const T ov = s_wv[par][q];
-                const int op = s_wp[par][q];
-                if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
-            }
-            // 2. publish: the holder of the workgroup's best row writes it,
-            //    its position, then the step tag (release)
-            const bool has = bp != INT_MAX;
-            if (has ? (active && pos == bp) : (t == 0)) {
-                const size_t slot = (size_t)par * gmax + g;
-                if (has) {
-                    T* dst = crow + slot * PW;
-#pragma unroll
-                    for (int c = j; c < PW; ++c) dst[c] = a[c];
-                }
-                w.cpos[slot] = has ? bp : -1;
-                __hip_atomic_store(&w.tag[slot], diag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            // 3. wave 0 waits for all G candidates, picks the pivot, stages
-            //    its row in LDS
+            // 2. wave 0: publish the workgroup's best, gather every
+            //    workgroup's, pick the pivot and stage its row
             if (wv == 0) {
+                T bv = s_wv[par][0];
+                int bp = s_wp[par][0], bw = 0;
+#pragma unroll
+                for (int q = 1; q < kW; ++q) {
+                    const T ov = s_wv[par][q];
+                    const int op = s_wp[par][q];
+                    if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; bw = q; }
+                }
+                const bool has = bp != INT_MAX;
+                unsigned long long* mine = words + ((size_t)par * gmax + g) * SW;
+                if (has) {
+                    for (int e = ln; e < (PW - j) * NW; e += kWave) {
+                        const int c = j + e / NW, q = e % NW;
+                        st_word(mine + 1 + c * NW + q, tag, piece_of(s_cand[par][bw][c], q));
+                    }
+                }
+                if (ln == 0) st_word(mine, tag, has ? (unsigned)bp : 0xFFFFFFFFu);
+                // every slot's position and current value
                 const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+                T cv = T(-1);
+                int cp = INT_MAX, cg = 0;
                 for (;;) {
-                    bool mine = true;
-                    for (int q = ln; q < G; q += kWave) mine = mine && (ld_agent(&w.tag[par * gmax + q]) == diag);
-                    if (__ballot(!mine) == 0ull) break;
+                    bool ok = true;
+                    cv = T(-1);
+                    cp = INT_MAX;
+                    cg = 0;
+                    for (int q = ln; q < G; q += kWave) {
+                        const unsigned long long* sl = words + ((size_t)par * gmax + q) * SW;
+                        const unsigned long long hw = ld_word(sl);
+                        if ((unsigned)(hw >> 32) != tag) { ok = false; continue; }
+                        const int p = (int)(unsigned)hw;
+                        if (p < 0) continue;                        // no candidate in that workgroup
+                        const unsigned long long w0 = ld_word(sl + 1 + j * NW);
+                        const unsigned long long w1 = NW == 2 ? ld_word(sl + 1 + j * NW + 1) : w0;
+                        if ((unsigned)(w0 >> 32) != tag || (unsigned)(w1 >> 32) != tag) { ok = false; continue; }
+                        T x = fabs(value_of<T>((unsigned)w0, (unsigned)w1));
+                        if (x != x) x = (p == diag) ? T(INFINITY) : T(-1);
+                        if (x >= T(0) && (x > cv || (x == cv && p < cp))) { cv = x; cp = p; cg = q; }
+                    }
+                    if (__ballot(!ok) == 0ull) break;
                     if (ld_agent(w.err) != 0) break;
                     if ((long long)(__builtin_amdgcn_s_memrealtime() - t0) > kDenseWaitTicks) {
                         if (ln == 0) __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                T cv = T(-1);
-                int cp = INT_MAX, cg = 0;
-                for (int q = ln; q < G; q += kWave) {
-                    const size_t slot = (size_t)par * gmax + q;
-                    const int p = w.cpos[slot];
-                    if (p >= 0) {
-                        T x = fabs(crow[slot * PW + j]);
-                        if (x != x) x = (p == diag) ? T(INFINITY) : T(-1);
-                        if (x >= T(0) && (x > cv || (x == cv && p < cp))) { cv = x; cp = p; cg = q; }
-                    }
                 }
                 const int mycp = cp;
                 argmax_t(cv, cp);
@@ -170,12 +221,30 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
                 // position in range then
                 int pp = cp;
                 if (pp < diag || pp >= m) pp = diag;
-                const T* src = crow + ((size_t)par * gmax + gw) * PW;
-                for (int c = ln; c < PW; c += kWave) s_u[par][c] = (hit != 0ull) ? src[c] : T(0);
+                // the winning row, from the current column on
+                const unsigned long long* wsl = words + ((size_t)par * gmax + gw) * SW;
+                const unsigned long long t1 = __builtin_amdgcn_s_memrealtime();
+                for (;;) {
+                    bool ok = true;
+                    for (int c = j + ln; c < PW; c += kWave) {
+                        const unsigned long long w0 = ld_word(wsl + 1 + c * NW);
+                        const unsigned long long w1 = NW == 2 ? ld_word(wsl + 1 + c * NW + 1) : w0;
+                        const bool good = hit != 0ull && (unsigned)(w0 >> 32) == tag && (unsigned)(w1 >> 32) == tag;
+                        ok = ok && (good || hit == 0ull);
+                        s_u[par][c] = good ? value_of<T>((unsigned)w0, (unsigned)w1) : T(0);
+                    }
+                    if (__ballot(!ok) == 0ull) break;
+                    if (ld_agent(w.err) != 0) break;
+                    if ((long long)(__builtin_amdgcn_s_memrealtime() - t1) > kDenseWaitTicks) {
+                        if (ln == 0) __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
                 if (ln == 0) s_pp[par] = pp;
             }
             __syncthreads();
-            // 4. interchange bookkeeping, scaling and rank-1 update
+            // 3. interchange bookkeeping, scaling and rank-1 update
             const int pp = s_pp[par];
             const T piv = s_u[par][j];
             if (g == 0 && t == 0) {
@@ -208,13 +277,14 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
 // Row pairs of the interchanges ipiv[s .. s+cnt) (LAPACK, 1-based, applied in
 // order): destination d receives the row found by tracing d back through
 // the interchanges. Destinations are the rows s .. s+cnt-1 and every
-// interchange partner below them (counted once). cnt <= 256.
+// interchange partner below them (counted once). cnt <= kDenseMaxNB, one
+// thread per candidate destination (2 * kDenseMaxNB threads).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(2 * kDenseMaxNB)
 dense_pivmap_kernel(const int* __restrict__ ipiv, int s, int cnt, int* __restrict__ pairs, int* __restrict__ npairs,
                     int maxp)
 {
-    __shared__ int piv[256];
+    __shared__ int piv[kDenseMaxNB];
     __shared__ int count;
     const int t = threadIdx.x;
     if (t == 0) count = 0;
@@ -615,6 +685,187 @@ dense_gemm_kernel(int M, int N, int K, const T* __restrict__ A, int lda, const T
     }
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit operands of the trailing update, converted once per update:
+//   Ah[i * ldk + k] = h(A21[i, k])   (rows with k contiguous, via an LDS tile)
+//   Bh[j * ldk + k] = h(U12[k, j])   (columns with k contiguous, as stored)
+// for k < ldk (K rounded up to 64; the padding is zero). With a split
+// (BF16x3) the second array of each holds the remainders lo = h(x - hi).
+// ---------------------------------------------------------------------------
+template <typename h_t>
+__global__ void __launch_bounds__(256)
+dense_cvt_a_kernel(int M, int K, int ldk, const float* __restrict__ A, int lda, h_t* __restrict__ hi,
+                   h_t* __restrict__ lo)
+{
+    __shared__ float tile[64][65];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int t = threadIdx.x;
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int i = e % 64, kk = e / 64;
+        tile[kk][i] = (i0 + i < M && k0 + kk < K) ? A[(size_t)(i0 + i) + (size_t)(k0 + kk) * lda] : 0.f;
+    }
+    __syncthreads();
+    typedef h_t v8h __attribute__((ext_vector_type(8)));
+    for (int e = t; e < 64 * 8; e += 256) {
+        const int i = e / 8, g = e % 8;
+        if (i0 + i >= M) continue;
+        v8h vh, vl;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const float x = tile[8 * g + v][i];
+            vh[v] = (h_t)x;
+            vl[v] = (h_t)(x - (float)vh[v]);
+        }
+        *reinterpret_cast<v8h*>(hi + (size_t)(i0 + i) * ldk + k0 + 8 * g) = vh;
+        if (lo) *reinterpret_cast<v8h*>(lo + (size_t)(i0 + i) * ldk + k0 + 8 * g) = vl;
+    }
+}
+
+template <typename h_t>
+__global__ void __launch_bounds__(256)
+dense_cvt_b_kernel(int N, int K, int ldk, const float* __restrict__ B, int ldb, h_t* __restrict__ hi,
+                   h_t* __restrict__ lo)
+{
+    const int e = blockIdx.x * 256 + threadIdx.x;          // (column, group of 8 k)
+    const int groups = ldk / 8;
+    const int j = e / groups, g = e % groups;
+    if (j >= N) return;
+    typedef h_t v8h __attribute__((ext_vector_type(8)));
+    v8h vh, vl;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const int kk = 8 * g + v;
+        const float x = kk < K ? B[(size_t)kk + (size_t)j * ldb] : 0.f;
+        vh[v] = (h_t)x;
+        vl[v] = (h_t)(x - (float)vh[v]);
+    }
+    *reinterpret_cast<v8h*>(hi + (size_t)j * ldk + 8 * g) = vh;
+    if (lo) *reinterpret_cast<v8h*>(lo + (size_t)j * ldk + 8 * g) = vl;
+}
+
+// C := C - A B on 128 x 128 tiles from the converted operands (Ah rows and
+// Bh columns with k contiguous, ld ldk, zero padded to a multiple of 64):
+// each stage moves 16-byte vectors straight into the LDS tiles.
+template <int MODE>
+__global__ void __launch_bounds__(256, 2)
+dense_gemm_h_kernel(int M, int N, int ldk, const void* __restrict__ Ahv, const void* __restrict__ Alv,
+                    const void* __restrict__ Bhv, const void* __restrict__ Blv, float* C, int ldc, int tiles_m,
+                    int tiles_n)
+{
+    using h_t = typename std::conditional<MODE == kUpdFP16, _Float16, __bf16>::type;
+    using v8_t = typename std::conditional<MODE == kUpdFP16, h8_t, bf8_t>::type;
+    constexpr int kSplit = (MODE == kUpdBF16x3) ? 2 : 1;
+    constexpr int BK = 64;
+    constexpr int LDK = BK + 8;
+    const h_t* Ah = static_cast<const h_t*>(Ahv);
+    const h_t* Bh = static_cast<const h_t*>(Bhv);
+    const h_t* Al = static_cast<const h_t*>(Alv);
+    const h_t* Bl = static_cast<const h_t*>(Blv);
+    const int tid = gemm_tile_id(tiles_m * tiles_n);
+    if (tid >= tiles_m * tiles_n) return;
+    const int m0 = (tid % tiles_m) * kGBM;
+    const int n0 = (tid / tiles_m) * kGBN;
+    const int t = threadIdx.x;
+    const int w = t / kWave, l = t % kWave;
+    const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+    const int lr = l % 16, lq = l / 16;
+    __shared__ __align__(16) h_t sA[kSplit][2][kGBM][LDK];
+    __shared__ __align__(16) h_t sB[kSplit][2][kGBN][LDK];
+    f4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4_t{0.f, 0.f, 0.f, 0.f};
+    // 128 rows x 8 groups of 8 k per operand tile: 4 vectors per thread
+    v8_t ra[kSplit][4], rb[kSplit][4];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const int r = e / 8, g = e % 8;
+            const bool va = m0 + r < M, vb = n0 + r < N;
+            const size_t oa = (size_t)(m0 + r) * ldk + k0 + 8 * g;
+            const size_t ob = (size_t)(n0 + r) * ldk + k0 + 8 * g;
+            ra[0][q] = va ? *reinterpret_cast<const v8_t*>(Ah + oa) : v8_t{};
+            rb[0][q] = vb ? *reinterpret_cast<const v8_t*>(Bh + ob) : v8_t{};
+            if constexpr (kSplit == 2) {
+                ra[1][q] = va ? *reinterpret_cast<const v8_t*>(Al + oa) : v8_t{};
+                rb[1][q] = vb ? *reinterpret_cast<const v8_t*>(Bl + ob) : v8_t{};
+            }
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            const int r = e / 8, g = e % 8;
+#pragma unroll
+            for (int h = 0; h < kSplit; ++h) {
+                *reinterpret_cast<v8_t*>(&sA[h][buf][r][8 * g]) = ra[h][q];
+                *reinterpret_cast<v8_t*>(&sB[h][buf][r][8 * g]) = rb[h][q];
+            }
+        }
+    };
+    const int nst = ldk / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nst) gload((s + 1) * BK);
+#pragma unroll
+        for (int kb = 0; kb < BK; kb += 32) {
+            v8_t pf[4], qf[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                pf[x] = *reinterpret_cast<const v8_t*>(&sB[0][buf][wc + 16 * x + lr][kb + 8 * lq]);
+                qf[x] = *reinterpret_cast<const v8_t*>(&sA[0][buf][wr + 16 * x + lr][kb + 8 * lq]);
+            }
+            if constexpr (MODE == kUpdBF16x3) {
+                v8_t pl[4], ql[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    pl[x] = *reinterpret_cast<const v8_t*>(&sB[1][buf][wc + 16 * x + lr][kb + 8 * lq]);
+                    ql[x] = *reinterpret_cast<const v8_t*>(&sA[1][buf][wr + 16 * x + lr][kb + 8 * lq]);
+                }
+#pragma unroll
+                for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                    for (int ti = 0; ti < 4; ++ti) {
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pl[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pf[tj], ql[ti], acc[tj][ti], 0, 0, 0);
+                    }
+            }
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti) {
+                    if constexpr (MODE == kUpdFP16)
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                    else
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                }
+        }
+        if (s + 1 < nst) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            const int row = m0 + wr + 16 * ti + lr;
+            if (row >= M) continue;
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) {
+                const int col = n0 + wc + 16 * tj + 4 * lq + rg;
+                if (col < N) {
+                    float* cp = C + (size_t)row + (size_t)col * ldc;
+                    *cp = *cp - acc[tj][ti][rg];
+                }
+            }
+        }
+}
+
 template <typename T>
 __global__ void dense_copy_kernel(int rows, int cols, const T* __restrict__ S, int lds, T* __restrict__ D, int ldd)
 {
@@ -644,35 +895,42 @@ int dense_panel_gmax(int m) { return (int)ceildiv(std::max(m, 1), kDensePanelRow
 
 static size_t a256(size_t b) { return (b + 255) & ~size_t(255); }
 
-size_t dense_work_bytes(int n, int elem, int gmax)
+size_t dense_work_bytes(int m, int n, int elem, int gmax)
 {
     const int PW = elem == 8 ? DenseShape<double>::PW : DenseShape<float>::PW;
-    const int NB = elem == 8 ? DenseShape<double>::NB : DenseShape<float>::NB;
+    const int NB = kDenseMaxNB;
     size_t b = 0;
-    b += a256(sizeof(int) * 2 * gmax);                  // tag
-    b += a256(sizeof(int) * 2 * gmax);                  // cpos
-    b += a256((size_t)elem * 2 * gmax * PW);            // crow
+    b += a256(sizeof(unsigned long long) * 2 * gmax * (1 + PW * elem / 4));   // slot words
     b += a256(sizeof(int) * 4);                         // err
     b += a256(sizeof(int) * 2 * 2 * NB);                // pairs
     b += a256(sizeof(int) * 4);                         // npairs
-    b += a256((size_t)elem * NB * NB);                  // X
-    if (elem == 4) b += a256((size_t)elem * NB * std::max(n, 1));   // W (FP32 only)
+    b += a256((size_t)elem * kDenseStrip * kDenseStrip);   // X
+    if (elem == 4) {                                          // Ah, Bh (hi and lo halves)
+        const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
+        b += 2 * a256(2 * 2 * h);
+    }
     return b;
 }
 
-void dense_work_carve(void* base, int n, int elem, int gmax, DenseWork& w)
+void dense_work_carve(void* base, int m, int n, int elem, int gmax, DenseWork& w)
 {
     const int PW = elem == 8 ? DenseShape<double>::PW : DenseShape<float>::PW;
-    const int NB = elem == 8 ? DenseShape<double>::NB : DenseShape<float>::NB;
+    const int NB = kDenseMaxNB;
     char* p = static_cast<char*>(base);
-    w.tag = reinterpret_cast<int*>(p);   p += a256(sizeof(int) * 2 * gmax);
-    w.cpos = reinterpret_cast<int*>(p);  p += a256(sizeof(int) * 2 * gmax);
-    w.crow = p;                          p += a256((size_t)elem * 2 * gmax * PW);
+    w.crow = p;                          p += a256(sizeof(unsigned long long) * 2 * gmax * (1 + PW * elem / 4));
+    w.nwords = 2 * gmax * (1 + PW * elem / 4);
     w.err = reinterpret_cast<int*>(p);   p += a256(sizeof(int) * 4);
     w.pairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 2 * 2 * NB);
     w.npairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 4);
-    w.X = p;                             p += a256((size_t)elem * NB * NB);
-    w.W = elem == 4 ? p : nullptr;
+    w.X = p;                             p += a256((size_t)elem * kDenseStrip * kDenseStrip);
+    w.Ah = w.Bh = nullptr;
+    w.h_elems = 0;
+    if (elem == 4) {
+        const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
+        w.Ah = p;                        p += a256(2 * 2 * h);
+        w.Bh = p;
+        w.h_elems = h;
+    }
     w.gmax = gmax;
 }
 
@@ -712,18 +970,44 @@ magma_int_t gemm_launch(int M, int N, int K, const T* A, int lda, const T* B, in
     return MAGMA_SUCCESS;
 }
 
+template <int MODE>
+magma_int_t update_h_launch(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
+                            const DenseWork& w, hipStream_t st)
+{
+    using h_t = typename std::conditional<MODE == kUpdFP16, _Float16, __bf16>::type;
+    if (M <= 0 || N <= 0) return MAGMA_SUCCESS;
+    const int ldk = (int)roundup(K, 64);
+    if (ldk > kDenseMaxNB || (size_t)M * ldk > w.h_elems || (size_t)N * ldk > w.h_elems) return MAGMA_ERR_NOT_SUPPORTED;
+    constexpr bool split = MODE == kUpdBF16x3;
+    h_t* Ah = static_cast<h_t*>(w.Ah);
+    h_t* Bh = static_cast<h_t*>(w.Bh);
+    h_t* Al = split ? Ah + w.h_elems : nullptr;
+    h_t* Bl = split ? Bh + w.h_elems : nullptr;
+    dense_cvt_a_kernel<h_t><<<dim3((unsigned)ceildiv(M, 64), (unsigned)(ldk / 64)), 256, 0, st>>>(M, K, ldk, A, lda,
+                                                                                                  Ah, Al);
+    dense_cvt_b_kernel<h_t><<<(unsigned)ceildiv((long)N * (ldk / 8), 256L), 256, 0, st>>>(N, K, ldk, B, ldb, Bh, Bl);
+    MGI_LAUNCH_CHECK();
+    const int tm = (int)ceildiv(M, kGBM), tn = (int)ceildiv(N, kGBN);
+    const int total = tm * tn;
+    const int grid = total >= 8 ? (int)roundup(total, 8) : total;
+    dense_gemm_h_kernel<MODE><<<grid, 256, 0, st>>>(M, N, ldk, Ah, Al, Bh, Bl, C, ldc, tm, tn);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 template <typename T>
 magma_int_t update_launch(int mode, int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
-                          hipStream_t st)
+                          const DenseWork& w, hipStream_t st)
 {
     if constexpr (std::is_same<T, double>::value) {
+        (void)w;
         return gemm_launch<double, kUpdF64, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
     } else {
         switch (mode) {
             case kUpdF32:    return gemm_launch<float, kUpdF32, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
-            case kUpdBF16:   return gemm_launch<float, kUpdBF16, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
-            case kUpdFP16:   return gemm_launch<float, kUpdFP16, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
-            case kUpdBF16x3: return gemm_launch<float, kUpdBF16x3, false>(M, N, K, A, lda, B, ldb, C, ldc, st);
+            case kUpdBF16:   return update_h_launch<kUpdBF16>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
+            case kUpdFP16:   return update_h_launch<kUpdFP16>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
+            case kUpdBF16x3: return update_h_launch<kUpdBF16x3>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
             default:         return MAGMA_ERR_NOT_SUPPORTED;
         }
     }
@@ -746,81 +1030,101 @@ template <typename T>
 magma_int_t swaps_launch(int m, const int* ipiv, int s, int cnt, T* A, int lda, const DenseWork& w, int c0, int c1,
                          int c2, int c3, hipStream_t st)
 {
-    constexpr int NB = DenseShape<T>::NB;
+    constexpr int MP = 2 * kDenseMaxNB;
     const int ncol = (c1 - c0) + (c3 - c2);
     if (ncol <= 0 || cnt <= 0) return MAGMA_SUCCESS;
-    dense_pivmap_kernel<<<1, 512, 0, st>>>(ipiv, s, cnt, w.pairs, w.npairs, 2 * NB);
+    if (cnt > kDenseMaxNB) return MAGMA_ERR_NOT_SUPPORTED;
+    dense_pivmap_kernel<<<1, 2 * kDenseMaxNB, 0, st>>>(ipiv, s, cnt, w.pairs, w.npairs, MP);
     MGI_LAUNCH_CHECK();
-    dense_laswp_kernel<T, 2 * NB / kWave><<<(unsigned)ceildiv(ncol, 4), 256, 0, st>>>(
-        m, A, lda, w.pairs, w.npairs, 2 * NB, c0, c1, c2, c3);
+    // at most 2 cnt pairs: the row-move kernel holds that many per wave
+    if (cnt <= 128)
+        dense_laswp_kernel<T, 4><<<(unsigned)ceildiv(ncol, 4), 256, 0, st>>>(m, A, lda, w.pairs, w.npairs, MP, c0, c1,
+                                                                            c2, c3);
+    else
+        dense_laswp_kernel<T, MP / kWave><<<(unsigned)ceildiv(ncol, 4), 256, 0, st>>>(m, A, lda, w.pairs, w.npairs, MP,
+                                                                                     c0, c1, c2, c3);
     MGI_LAUNCH_CHECK();
     return MAGMA_SUCCESS;
 }
 
-// U12 := L11^{-1} A[k:k+jb, cb:ce) (jb x jb unit lower L11 at (k,k)), then
-// A[k+jb:m, cb:ce) -= A[k+jb:m, k:k+jb) U12. FP64 (jb <= 128 = one GEMM row
-// tile): each workgroup of the U12 product owns all jb rows of its columns
-// and writes them only after reading them, so the product runs in place.
-// FP32 (jb up to 256, two row tiles): U12 goes through W, then into A.
+// U12 := L11^{-1} A[r0:r0+kw, cb:ce) for the unit lower L11 at (r0, r0),
+// then A[r0+kw:m, cb:ce) -= A[r0+kw:m, r0:r0+kw) U12. The triangular solve
+// runs in strips of up to 128 rows: strip s is multiplied by the inverse of
+// its diagonal block (one GEMM row tile: each workgroup owns all rows of its
+// columns and writes them after reading them, so the product runs in place),
+// then removed from the strips below (a GEMM update). The triangular part
+// runs in the storage precision (FP64, or FP32 as MAGMA's strsm); the
+// trailing update in the requested format.
 template <typename T>
-magma_int_t trsm_update(int mode, int m, int k, int jb, int cb, int ce, T* A, int lda, const DenseWork& w,
+magma_int_t trsm_update(int mode, int m, int r0, int kw, int cb, int ce, T* A, int lda, const DenseWork& w,
                         hipStream_t st)
 {
-    constexpr int NB = DenseShape<T>::NB;
     const int nc = ce - cb;
-    if (nc <= 0 || jb <= 0) return MAGMA_SUCCESS;
+    if (nc <= 0 || kw <= 0) return MAGMA_SUCCESS;
     T* X = reinterpret_cast<T*>(w.X);
-    magma_int_t rc = dense_trtri_launch<T>(false, k + jb, k, NB, 1, A, lda, X, st);
-    if (rc != MAGMA_SUCCESS) return rc;
-    T* A12 = A + k + (size_t)cb * lda;
-    const T* U12 = A12;
-    int ldu = lda;
-    if constexpr (std::is_same<T, double>::value) {
-        static_assert(DenseShape<double>::NB <= kGBM, "in-place U12 needs one row tile");
-        rc = gemm_launch<double, kUpdF64, true>(jb, nc, jb, X, NB, A12, lda, A12, lda, st);
+    constexpr int S = kDenseStrip;
+    static_assert(S <= kGBM, "in-place strips need one GEMM row tile");
+    constexpr int kTri = std::is_same<T, double>::value ? (int)kUpdF64 : (int)kUpdF32;
+    magma_int_t rc = MAGMA_SUCCESS;
+    for (int s0 = 0; s0 < kw; s0 += S) {
+        const int ws = std::min(S, kw - s0);
+        const int rs = r0 + s0;
+        if ((rc = dense_trtri_launch<T>(false, rs + ws, rs, S, 1, A, lda, X, st)) != MAGMA_SUCCESS) return rc;
+        T* Us = A + rs + (size_t)cb * lda;
+        if constexpr (kTri == kUpdF64) rc = gemm_launch<double, kUpdF64, true>(ws, nc, ws, X, S, Us, lda, Us, lda, st);
+        else rc = gemm_launch<float, kUpdF32, true>(ws, nc, ws, X, S, Us, lda, Us, lda, st);
         if (rc != MAGMA_SUCCESS) return rc;
-    } else {
-        T* W = reinterpret_cast<T*>(w.W);
-        rc = gemm_launch<float, kUpdF32, true>(jb, nc, jb, X, NB, A12, lda, W, NB, st);
-        if (rc != MAGMA_SUCCESS) return rc;
-        dense_copy_kernel<T><<<dim3((unsigned)ceildiv(jb, 256), (unsigned)nc), 256, 0, st>>>(jb, nc, W, NB, A12,
-                                                                                               lda);
-        MGI_LAUNCH_CHECK();
-        U12 = W;
-        ldu = NB;
+        const int below = kw - s0 - ws;
+        if (below > 0) {
+            if constexpr (kTri == kUpdF64)
+                rc = gemm_launch<double, kUpdF64, false>(below, nc, ws, A + rs + ws + (size_t)rs * lda, lda, Us, lda,
+                                                         A + rs + ws + (size_t)cb * lda, lda, st);
+            else
+                rc = gemm_launch<float, kUpdF32, false>(below, nc, ws, A + rs + ws + (size_t)rs * lda, lda, Us, lda,
+                                                        A + rs + ws + (size_t)cb * lda, lda, st);
+            if (rc != MAGMA_SUCCESS) return rc;
+        }
     }
-    if (m - k - jb > 0)
-        rc = update_launch<T>(mode, m - k - jb, nc, jb, A + (k + jb) + (size_t)k * lda, lda, U12, ldu,
-                              A + (k + jb) + (size_t)cb * lda, lda, st);
+    if (m - r0 - kw > 0)
+        rc = update_launch<T>(mode, m - r0 - kw, nc, kw, A + (r0 + kw) + (size_t)r0 * lda, lda, A + r0 + (size_t)cb * lda,
+                              lda, A + (r0 + kw) + (size_t)cb * lda, lda, w, st);
     return rc;
 }
 
+// Blocks of nb columns (a multiple of the panel width PW), each factored as
+// nb / PW panels: after panel i, its interchanges go to the block's other
+// columns and its U rows and update to the block's columns on its right;
+// after the block, its interchanges go to every column outside it and its
+// U12 and trailing update (K = nb) to the columns on its right.
 template <typename T>
-magma_int_t dense_getrf_t(int mode, int m, int n, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
+magma_int_t dense_getrf_t(int mode, int nb, int m, int n, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
                           hipStream_t st)
 {
     constexpr int PW = DenseShape<T>::PW;
-    constexpr int NB = DenseShape<T>::NB;
+    if (nb < PW || nb % PW != 0) return MAGMA_ERR_NOT_SUPPORTED;
     if (dense_panel_gmax(m) > w.gmax) return MAGMA_ERR_NOT_SUPPORTED;
-    dense_fill_int_kernel<<<(unsigned)ceildiv(2 * w.gmax, 256), 256, 0, st>>>(w.tag, -1, 2 * w.gmax);
-    dense_fill_int_kernel<<<(unsigned)ceildiv(2 * w.gmax, 256), 256, 0, st>>>(w.cpos, -1, 2 * w.gmax);
+    // slot words with an impossible tag: nothing of an earlier call is accepted
+    dense_fill_int_kernel<<<(unsigned)ceildiv(2 * w.nwords, 256), 256, 0, st>>>(static_cast<int*>(w.crow), -1,
+                                                                               2 * w.nwords);
     dense_fill_int_kernel<<<1, 32, 0, st>>>(w.err, 0, 1);
     dense_fill_int_kernel<<<1, 32, 0, st>>>(info, 0, 1);
     MGI_LAUNCH_CHECK();
     const int minmn = std::min(m, n);
     magma_int_t rc = MAGMA_SUCCESS;
-    for (int k = 0; k < minmn; k += NB) {
-        const int jb = std::min(NB, minmn - k);
-        const int jb1 = std::min(PW, jb);
-        if ((rc = panel_launch<T>(m, k, jb1, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
-        if (jb > jb1) {
-            const int jb2 = jb - jb1;
-            if ((rc = swaps_launch<T>(m, ipiv, k, jb1, A, lda, w, k + jb1, k + jb, 0, 0, st)) != MAGMA_SUCCESS)
-                return rc;
-            if ((rc = trsm_update<T>(mode, m, k, jb1, k + jb1, k + jb, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
-            if ((rc = panel_launch<T>(m, k + jb1, jb2, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
-            if ((rc = swaps_launch<T>(m, ipiv, k + jb1, jb2, A, lda, w, k, k + jb1, 0, 0, st)) != MAGMA_SUCCESS)
-                return rc;
+    for (int k = 0; k < minmn; k += nb) {
+        const int jb = std::min(nb, minmn - k);
+        for (int i0 = 0; i0 < jb; i0 += PW) {
+            const int wdt = std::min(PW, jb - i0);
+            const int kp = k + i0;
+            if ((rc = panel_launch<T>(m, kp, wdt, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+            const int right = kp + wdt;
+            if (i0 > 0 || right < k + jb) {
+                if ((rc = swaps_launch<T>(m, ipiv, kp, wdt, A, lda, w, k, kp, right, k + jb, st)) != MAGMA_SUCCESS)
+                    return rc;
+            }
+            if (right < k + jb) {
+                if ((rc = trsm_update<T>(mode, m, kp, wdt, right, k + jb, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+            }
         }
         if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w, 0, k, k + jb, n, st)) != MAGMA_SUCCESS) return rc;
         if ((rc = trsm_update<T>(mode, m, k, jb, k + jb, n, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
@@ -831,18 +1135,19 @@ magma_int_t dense_getrf_t(int mode, int m, int n, T* A, int lda, int* ipiv, int*
 }  // namespace
 
 magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
-                        hipStream_t stream)
+                        hipStream_t stream, int nb)
 {
     (void)mode;
-    return dense_getrf_t<double>(kUpdF64, m, n, dA, ldda, d_ipiv, d_info, w, stream);
+    return dense_getrf_t<double>(kUpdF64, nb > 0 ? nb : DenseShape<double>::NB, m, n, dA, ldda, d_ipiv, d_info, w,
+                                 stream);
 }
 
 magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
-                         hipStream_t stream)
+                         hipStream_t stream, int nb)
 {
     if (mode != kUpdF32 && mode != kUpdBF16 && mode != kUpdFP16 && mode != kUpdBF16x3)
         return MAGMA_ERR_NOT_SUPPORTED;
-    return dense_getrf_t<float>(mode, m, n, dA, ldda, d_ipiv, d_info, w, stream);
+    return dense_getrf_t<float>(mode, nb > 0 ? nb : DenseShape<float>::NB, m, n, dA, ldda, d_ipiv, d_info, w, stream);
 }
 
 magma_int_t dense_check_err(const DenseWork& w, hipStream_t stream)

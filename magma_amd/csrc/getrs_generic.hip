@@ -39,7 +39,7 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     const int tid = threadIdx.x;
     const int nr = min(NCOLS, nrhs - c0);
     const T* A = dA_array[b];
-    const int* ipiv = ipiv_array[b];
+    const int* ipiv = ipiv_array ? ipiv_array[b] : nullptr;   // null: factors without pivoting
     T* B = dB_array[b] + (size_t)c0 * lddb;
 
     // (P^T B)(i,:) = B(perm[i],:), P the product of the interchanges
@@ -48,7 +48,7 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     // [k, n) cannot come from a valid LU and is treated as none.
     int* spiv = perm + sz.max_n;
     for (int i = tid; i < n; i += kRsThreads) {
-        const int p = ipiv[i] - 1;
+        const int p = ipiv ? ipiv[i] - 1 : i;
         spiv[i] = (p > i && p < n) ? p : i;
     }
     __syncthreads();
@@ -217,14 +217,14 @@ getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, i
     const int nr = min(kWaveCols, nrhs - c0);
     const int l = threadIdx.x % kWave;
     const T* A = dA_array[b];
-    const int* ipiv = ipiv_array[b];
+    const int* ipiv = ipiv_array ? ipiv_array[b] : nullptr;   // null: factors without pivoting
     T* B = dB_array[b] + (size_t)c0 * lddb;
 
     // (P^T B)(i,:) = B(src(i),:)
     int src[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) src[s] = l + 64 * s;
-    for (int k = n - 1; k >= 0; --k) {
+    for (int k = ipiv ? n - 1 : -1; k >= 0; --k) {
         int p = ipiv[k] - 1;
         p = (p > k && p < n) ? p : k;
 #pragma unroll

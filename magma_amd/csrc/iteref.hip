@@ -33,7 +33,12 @@ namespace mgi {
 constexpr int kIrThreads = 256;
 // single systems above this order go to the dense LU on the whole device
 constexpr int kDenseIrMinN = 512;
-constexpr int kGmresMaxRestart = 50;   // restrt = inner_itermax, src/dxgesv_gmres_gpu.cpp:212
+// Restart of the FGMRES: one system restarts after ITERMAX = min(n, 500)
+// iterations as src/dxgesv_gmres_gpu.cpp:574-575 does; the batched solver
+// restarts after 50 (inner_itermax, :212) to bound the Krylov basis of a
+// whole batch (n x 101 doubles per system instead of n x 1001).
+constexpr int kGmresMaxRestart = 500;
+constexpr int kGmresBatchedRestart = 50;
 constexpr int kGmresCheck = 4;       // inner steps between host checks of the open cycles
 
 struct IrState {
@@ -501,12 +506,12 @@ static T* carve(char*& p, size_t count)
 // iter_out[s]: >= 0 refinement steps, -2 overflow, -3 low-precision LU failed,
 // -k not converged after k steps. info_out[s]: FP64 fallback info.
 magma_int_t iteref_batched(int trans, int n, int nrhs, double** dA_array, int ldda, double** dB_array, int lddb,
-                           double** dX_array, int lddx, int mode, bool gmres, int itermax, int* iter_out,
-                           int* info_out, int** lp_ipiv_out, int batch, magma_queue_t queue,
+                           double** dX_array, int lddx, int mode, bool gmres, int itermax, int restart,
+                           int* iter_out, int* info_out, int** lp_ipiv_out, int batch, magma_queue_t queue,
                            float* ws_float /* nullable: n*n*batch floats */)
 {
     hipStream_t stream = queue->stream;
-    const int m = std::max(1, std::min({kGmresMaxRestart, itermax, n}));
+    const int m = std::max(1, std::min({kGmresMaxRestart, restart, itermax, n}));
     // workspace layout
     size_t bytes = 0;
     auto add = [&](size_t b) { bytes += (b + 255) & ~size_t(255); };
@@ -744,8 +749,8 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs, double*
     if (!getrf_blocked_supported(n)) return MAGMA_ERR_NOT_SUPPORTED;
     hipSetDevice(queue->device);
     return iteref_batched(0, n, nrhs, dA_array, ldda, dB_array, lddb, dX_array, lddx, mode,
-                          solver_type == Magma_REFINE_GMSTRS, itermax, iter_array, info_array, nullptr, batchCount,
-                          queue, nullptr);
+                          solver_type == Magma_REFINE_GMSTRS, itermax, kGmresBatchedRestart, iter_array, info_array,
+                          nullptr, batchCount, queue, nullptr);
 }
 
 // src/dxgesv_gmres_gpu.cpp:160 (expert driver) on one system.
@@ -767,19 +772,21 @@ magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int
     else if (lddb < std::max(1, n)) *info = -9;
     else if (lddx < std::max(1, n)) *info = -11;
     else if (mode < 0) *info = -14;
-    else if (solver_type != Magma_REFINE_IRSTRS && solver_type != Magma_REFINE_GMSTRS) *info = -15;
+    else if (solver_type < Magma_REFINE_IRSTRS || solver_type > Magma_REFINE_GMGMDTRS) *info = -15;
     if (*info != 0) {
         magma_xerbla(__func__, -(*info));
         return *info;
     }
     if (n == 0 || nrhs == 0) return 0;
-    if (n > kDenseIrMinN || !getrf_blocked_supported(n)) {
-        // one large system: the dense LU on the whole device
+    const bool s_plain = solver_type == Magma_REFINE_IRSTRS || solver_type == Magma_REFINE_GMSTRS;
+    if (!s_plain || n > kDenseIrMinN || !getrf_blocked_supported(n)) {
+        // one large system, or a mode the batched engine does not run: the
+        // dense LU on the whole device
         const double t0 = magma_wtime();
         int it = 0, inf = 0;
         const magma_int_t rc = dense_iteref(trans == MagmaNoTrans ? 0 : 1, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb,
-                                            dX, lddx, dworkd, dworks, mode, solver_type == Magma_REFINE_GMSTRS, &it,
-                                            &inf);
+                                            dX, lddx, dworkd, dworks, mode, (int)(solver_type - Magma_REFINE_IRSTRS),
+                                            &it, &inf);
         if (facto_time) *facto_time = magma_wtime() - t0;
         if (rc != MAGMA_SUCCESS) {
             *info = rc;
@@ -809,8 +816,8 @@ magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int
         int* lp_ipiv = nullptr;
         // The caller's FP32 workspace (n*(n+nrhs) floats) holds the low-precision LU.
         rc = iteref_batched(trans == MagmaNoTrans ? 0 : 1, n, nrhs, ptrs, ldda, ptrs + 1, lddb, ptrs + 2, lddx, mode,
-                            solver_type == Magma_REFINE_GMSTRS, std::min(n, 500), small, small + 1, &lp_ipiv, 1, q,
-                            dworks);
+                            solver_type == Magma_REFINE_GMSTRS, std::min(n, 500), std::min(n, 500), small,
+                            small + 1, &lp_ipiv, 1, q, dworks);
         if (rc == MAGMA_SUCCESS) {
             int h[2];
             hipMemcpy(h, small, sizeof(h), hipMemcpyDeviceToHost);
@@ -848,6 +855,17 @@ magma_int_t magma_dhgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
     real_Double_t ft;
     return magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
                                   Magma_PREC_SHT, Magma_REFINE_GMSTRS, iter, info, &ft);
+}
+
+// src/dxgesv_gmres_gpu.cpp:966: FP32 LU + FGMRES.
+magma_int_t magma_dsgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs, magmaDouble_ptr dA,
+                                    magma_int_t ldda, magma_int_t* ipiv, magmaInt_ptr dipiv, magmaDouble_ptr dB,
+                                    magma_int_t lddb, magmaDouble_ptr dX, magma_int_t lddx, magmaDouble_ptr dworkd,
+                                    magmaFloat_ptr dworks, magma_int_t* iter, magma_int_t* info)
+{
+    real_Double_t ft;
+    return magma_dxgesv_gmres_gpu(trans, n, nrhs, dA, ldda, ipiv, dipiv, dB, lddb, dX, lddx, dworkd, dworks,
+                                  Magma_PREC_SS, Magma_REFINE_GMSTRS, iter, info, &ft);
 }
 
 magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_int_t nrhs, magmaDouble_ptr dA,

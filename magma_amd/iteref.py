@@ -15,7 +15,13 @@ FACTO = {
     "bf16": _ffi.Magma_PREC_SBT,
     "bf16x3": _ffi.Magma_PREC_SB3T,
 }
-SOLVER = {"ir": _ffi.Magma_REFINE_IRSTRS, "gmres": _ffi.Magma_REFINE_GMSTRS}
+# short names of the refinement solvers (include/magma_types.h Magma_REFINE_*):
+# "ir" IRSTRS, "gmres" GMSTRS, and the reference's own names in lower case
+SOLVER = {"ir": _ffi.Magma_REFINE_IRSTRS, "gmres": _ffi.Magma_REFINE_GMSTRS,
+          "irstrs": _ffi.Magma_REFINE_IRSTRS, "irdtrs": _ffi.Magma_REFINE_IRDTRS,
+          "irgmstrs": _ffi.Magma_REFINE_IRGMSTRS, "irgmdtrs": _ffi.Magma_REFINE_IRGMDTRS,
+          "gmstrs": _ffi.Magma_REFINE_GMSTRS, "gmdtrs": _ffi.Magma_REFINE_GMDTRS,
+          "gmgmstrs": _ffi.Magma_REFINE_GMGMSTRS, "gmgmdtrs": _ffi.Magma_REFINE_GMGMDTRS}
 
 
 def dxgesv_iteref_batched(A: torch.Tensor, B: torch.Tensor, X: torch.Tensor | None = None, facto: str = "bf16",
@@ -56,7 +62,11 @@ def dhgesv_iteref_gpu(A: torch.Tensor, B: torch.Tensor, facto: str = "fp16", sol
     it, info = ctypes.c_int(0), ctypes.c_int(0)
     lib = _ffi.load()
     torch.cuda.synchronize(A.device)
-    if facto == "fp16" and solver == "gmres":
+    if facto == "fp32" and solver == "gmres":
+        rc = lib.magma_dsgesv_iteref_gpu(trans, n, nrhs, A.data_ptr(), ldda, ipiv, dipiv.data_ptr(), B.data_ptr(),
+                                         lddb, X.data_ptr(), lddb, dworkd.data_ptr(), dworks.data_ptr(),
+                                         ctypes.byref(it), ctypes.byref(info))
+    elif facto == "fp16" and solver == "gmres":
         rc = lib.magma_dhgesv_iteref_gpu(trans, n, nrhs, A.data_ptr(), ldda, ipiv, dipiv.data_ptr(), B.data_ptr(),
                                          lddb, X.data_ptr(), lddb, dworkd.data_ptr(), dworks.data_ptr(),
                                          ctypes.byref(it), ctypes.byref(info))

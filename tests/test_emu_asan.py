@@ -38,3 +38,20 @@ def test_kernels_under_asan(emu_lib):
     assert "AddressSanitizer" not in out, out[-6000:]
     assert r.returncode == 0, out[-6000:]
     assert "all scenarios passed" in out
+
+
+# The single-matrix LU's panel is a cooperative kernel whose workgroups wait
+# on one another, so the emulation runs several workgroups interleaved.
+DENSE_SCENARIOS = ["dgetrf:300:280", "dgetrf:600:700", "sgetrf:520:530", "sgetrfp:200:200:bf16x3",
+                   "sgetrfp:300:260:fp16", "sgetrfp:260:300:bf16", "dgetrs:300:3:0", "dgetrs:300:3:1"]
+
+
+@pytest.mark.timeout(900)
+def test_dense_kernels_under_asan(emu_lib):
+    env = dict(os.environ, LD_PRELOAD=emu_lib, ASAN_OPTIONS="detect_leaks=0", EMU_WG_WINDOW="5")
+    r = subprocess.run([sys.executable, str(HERE / "emu" / "scenarios.py"), *DENSE_SCENARIOS], env=env,
+                       capture_output=True, text=True, timeout=880)
+    out = r.stdout + r.stderr
+    assert "AddressSanitizer" not in out, out[-6000:]
+    assert r.returncode == 0, out[-6000:]
+    assert "all scenarios passed" in out

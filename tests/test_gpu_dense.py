@@ -207,29 +207,112 @@ def test_dense_refinement_2048(dn, oracle, facto, solver, cond):
     print(f"n={n} {facto}-{solver} cond={cond:.0e}: iter {it} be {be:.2e} fe {fe:.2e}")
 
 
-@pytest.mark.timeout(600)
+def MAGMA_BWD(n):
+    """Backward error the refinement's stopping test guarantees:
+    ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n)
+    (src/dxgesv_gmres_gpu.cpp:490-493, eps = dlamch('Epsilon') = 2^-53);
+    below 1e-14 for n <= 8,192."""
+    return 2.0 ** -53 * np.sqrt(n)
+
+
+MODES = ["irstrs", "irdtrs", "irgmstrs", "irgmdtrs", "gmstrs", "gmdtrs", "gmgmstrs", "gmgmdtrs"]
+
+
+@pytest.mark.parametrize("solver", MODES)
+@pytest.mark.parametrize("n,facto,cond,trans", [(300, "fp32", 1e4, 0), (1500, "fp16", 1e4, 0), (777, "bf16", 1e2, 1)])
+def test_dense_refinement_modes(dn, oracle, solver, n, facto, cond, trans):
+    """Every solver type of src/dxgesv_gmres_gpu.cpp:328-359: the answer
+    meets 1e-14 normwise backward error (or the tester's bound after the FP64
+    fallback) and matches the oracle's FP64 solve; iterations are counted as
+    the reference counts them (inner iterations for the nested modes)."""
+    from magma_amd import iteref
+    A = svd_geo_gpu(n, cond, seed=n)
+    rng = np.random.default_rng(n + len(solver))
+    b = torch.from_numpy(rng.uniform(-1, 1, (2, n))).cuda()
+    dA = A.clone()
+    from magma_amd import _ffi
+    X, ipiv, dipiv, it, info = iteref.dhgesv_iteref_gpu(dA, b.clone(), facto=facto, solver=solver,
+                                                        trans=_ffi.MagmaTrans if trans else _ffi.MagmaNoTrans)
+    assert info == 0
+    At = A if trans == 0 else A.T.contiguous()        # column storage of op(A)
+    be = normwise_be(At, X, b)
+    if it >= 0:
+        assert be <= 1e-14, (it, be)
+    else:
+        assert be <= TESTER_TOL * n, (it, be)
+    Xo = oracle.dgesv(At.cpu().numpy()[None], b.cpu().numpy()[None])[3][0]
+    fe = float(np.abs(X.cpu().numpy() - Xo).max() / np.abs(Xo).max())
+    assert fe <= max(cond, 10.0) * 1e-13, fe
+    # the low-precision LU's pivots come back in ipiv and their row map in dipiv
+    if it >= 0:
+        perm = np.arange(n)
+        for k, p in enumerate(ipiv):
+            assert 1 <= p <= n
+            perm[[k, p - 1]] = perm[[p - 1, k]]
+        dp = dipiv.cpu().numpy()
+        assert np.array_equal(dp, perm if trans == 0 else np.argsort(perm))
+    print(f"n={n} {facto}-{solver} cond={cond:.0e} trans={trans}: iter {it} be {be:.2e} fe {fe:.2e}")
+
+
+def test_dsgesv_iteref_gpu(dn, oracle):
+    """magma_dsgesv_iteref_gpu (src/dxgesv_gmres_gpu.cpp:966): FP32 LU +
+    FGMRES, at a size of the batched engine and one of the dense LU."""
+    from magma_amd import iteref
+    for n in (200, 1200):
+        A = svd_geo_gpu(n, 1e5, seed=3)
+        b = torch.from_numpy(np.random.default_rng(n).uniform(-1, 1, (3, n))).cuda()
+        X, ipiv, dipiv, it, info = iteref.dhgesv_iteref_gpu(A.clone(), b.clone(), facto="fp32", solver="gmres")
+        assert info == 0 and it >= 0, (n, it, info)
+        assert normwise_be(A, X, b) <= 1e-14
+        Xo = oracle.dgesv(A.cpu().numpy()[None], b.cpu().numpy()[None])[3][0]
+        assert float(np.abs(X.cpu().numpy() - Xo).max() / np.abs(Xo).max()) <= 1e5 * 1e-13
+
+
+@pytest.mark.timeout(900)
 def test_dense_refinement_full_size(dn):
-    """n = 24,576 (the dense half of the mixed-precision workload): FP64-level
-    backward error and forward error against the FP64 GPU solve
-    (magma_dgetrf_gpu + magma_dgetrs_gpu)."""
+    """n = 24,576 (the dense half of the mixed-precision workload), condition
+    1e2, 1e4, 1e6, BF16 and FP16 updates, classical and GMRES refinement:
+    FP64-level backward error (or the FP64 fallback with the tester's bound)
+    and forward error against the FP64 GPU solve (magma_dgetrf_gpu +
+    magma_dgetrs_gpu)."""
     n = 24576
-    for cond, facto, solver in ((1e2, "bf16", "ir"), (1e4, "bf16", "gmres"), (1e6, "fp16", "gmres")):
-        A = svd_geo_gpu(n, cond, seed=7)
-        rng = np.random.default_rng(1)
-        b = torch.from_numpy(rng.uniform(-1, 1, (1, n))).cuda()
-        X, it, info, _ = dense_refine(A, b, facto, solver)
-        assert info == 0
-        be = normwise_be(A, X, b)
-        assert (it >= 0 and be <= 1e-14) or (it < 0 and be <= TESTER_TOL * n), (cond, facto, solver, it, be)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+
+    def haar():
+        Z = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        Q, R = torch.linalg.qr(Z)
+        del Z
+        return Q * torch.sign(torch.diagonal(R))[None, :]
+    U = haar()
+    V = haar()
+    rng = np.random.default_rng(1)
+    b = torch.from_numpy(rng.uniform(-1, 1, (1, n))).cuda()
+    for cond in (1e2, 1e4, 1e6):
+        sig = cond ** (-torch.arange(n, dtype=torch.float64, device="cuda") / (n - 1))
+        A = ((U * sig[None, :]) @ V.T.contiguous()).T.contiguous()
         LU = A.clone()
         ipiv, info = dn.getrf_gpu(LU)
         assert info == 0
         X64 = b.clone()
         assert dn.getrs_gpu(LU, ipiv, X64) == 0
+        del LU
         be64 = normwise_be(A, X64, b)
         assert be64 <= TESTER_TOL * n
-        fe = float((X - X64).abs().max() / X64.abs().max())
-        assert fe <= cond * 1e-12, fe
-        print(f"n={n} {facto}-{solver} cond={cond:.0e}: iter {it} be {be:.2e} fp64 be {be64:.2e} fe {fe:.2e}")
-        del A, LU
+        for facto in ("bf16", "fp16"):
+            for solver in ("ir", "gmres"):
+                X, it, info, dA = dense_refine(A, b, facto, solver)
+                del dA
+                assert info == 0
+                be = normwise_be(A, X, b)
+                # MAGMA stops at ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n): a
+                # backward error of at most eps sqrt(n) = 1.74e-14 at this order
+                assert (it >= 0 and be <= MAGMA_BWD(n)) or (it < 0 and be <= TESTER_TOL * n), \
+                    (cond, facto, solver, it, be)
+                fe = float((X - X64).abs().max() / X64.abs().max())
+                assert fe <= cond * 1e-12, fe
+                print(f"n={n} {facto}-{solver} cond={cond:.0e}: iter {it} be {be:.2e} fp64 be {be64:.2e} "
+                      f"fe {fe:.2e}", flush=True)
+                torch.cuda.empty_cache()
+        del A
         torch.cuda.empty_cache()
