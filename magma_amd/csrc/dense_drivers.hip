@@ -364,6 +364,9 @@ __global__ void dn_zero_i_kernel(int* p, int count)
 // --------------------------------------------------------------------------
 struct DevBuf {
     void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { if (p) (void)hipFree(p); }
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 256)); }
 };
@@ -392,8 +395,11 @@ magma_int_t getrf_one(int mode, int m, int n, T* dA, int ldda, int* h_ipiv, int*
     int* d_info = static_cast<int*>(small.p);
     int* d_ipiv = d_info + 64;
     magma_int_t rc;
-    if constexpr (std::is_same<T, double>::value) rc = dense_getrf(kUpdF64, m, n, dA, ldda, d_ipiv, d_info, w, st);
-    else rc = dense_sgetrf(mode, m, n, dA, ldda, d_ipiv, d_info, w, st);
+    StreamHolder side;
+    side.s = dense_side_stream(m, (int)sizeof(T));
+    if constexpr (std::is_same<T, double>::value)
+        rc = dense_getrf(kUpdF64, m, n, dA, ldda, d_ipiv, d_info, w, st, 0, side.s);
+    else rc = dense_sgetrf(mode, m, n, dA, ldda, d_ipiv, d_info, w, st, 0, side.s);
     if (rc != MAGMA_SUCCESS) return rc;
     if ((rc = dense_check_err(w, st)) != MAGMA_SUCCESS) return rc;
     MGI_HIP_CHECK(hipMemcpy(h_ipiv, d_ipiv, sizeof(int) * minmn, hipMemcpyDeviceToHost));
@@ -515,6 +521,14 @@ static magma_int_t ir_residual(DenseIrCtx& c, const double* B, int ldb, const do
 // of the same factors (magma_dgetrs_gpu on slag2d of them).
 static magma_int_t ir_trs(DenseIrCtx& c, const double* Sv, int ldsv, int nq, double* D, int ldd, int acc)
 {
+    // at most the solve scratch's column count per pass
+    const int cap = c.dtrs ? c.swd.nrhs_max : c.sw.nrhs_max;
+    for (int q0 = cap; q0 < nq; q0 += cap) {
+        const magma_int_t rc = ir_trs(c, Sv + (size_t)q0 * ldsv, ldsv, std::min(cap, nq - q0), D + (size_t)q0 * ldd,
+                                      ldd, acc);
+        if (rc != MAGMA_SUCCESS) return rc;
+    }
+    nq = std::min(nq, cap);
     const int n = c.n;
     dim3 g(gx(n), (unsigned)nq);
     magma_int_t rc;
@@ -540,25 +554,30 @@ static bool col_ok(double r, double x, double cte)
 }
 
 // Krylov workspace of one FGMRES level: V n x (m+1), Z n x m, r n, and the
-// reduction buffers.
+// reduction buffers (pointers into the memory a GmOwner holds).
 struct GmBuf {
-    DevBuf vb, hb;
     double* V = nullptr; double* Z = nullptr; double* r = nullptr;
     double* part = nullptr; double* hd = nullptr; double* hacc = nullptr; double* nrm2 = nullptr;
     int m = 0;
+};
+
+struct GmOwner {
+    DevBuf vb, hb;
+    GmBuf g;
     hipError_t alloc(int n, int restart)
     {
-        m = restart;
+        const int m = restart;
         hipError_t e = vb.alloc(sizeof(double) * (size_t)n * (2 * m + 2));
         if (e != hipSuccess) return e;
         if ((e = hb.alloc(sizeof(double) * ((size_t)kDotChunks * (m + 2) + 4 * (m + 2)))) != hipSuccess) return e;
-        V = static_cast<double*>(vb.p);
-        Z = V + (size_t)n * (m + 1);
-        r = Z + (size_t)n * m;
-        part = static_cast<double*>(hb.p);           // kDotChunks x (m+2)
-        hd = part + (size_t)kDotChunks * (m + 2);    // m+2: h of the current pass
-        hacc = hd + (m + 2);                         // m+2: h summed over the two passes
-        nrm2 = hacc + (m + 2);                       // [0] squared norm
+        g.m = m;
+        g.V = static_cast<double*>(vb.p);
+        g.Z = g.V + (size_t)n * (m + 1);
+        g.r = g.Z + (size_t)n * m;
+        g.part = static_cast<double*>(hb.p);             // kDotChunks x (m+2)
+        g.hd = g.part + (size_t)kDotChunks * (m + 2);    // m+2: h of the current pass
+        g.hacc = g.hd + (m + 2);                         // m+2: h summed over the two passes
+        g.nrm2 = g.hacc + (m + 2);                       // [0] squared norm
         return hipSuccess;
     }
 };
@@ -849,7 +868,9 @@ magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* 
         fallback = true;
     } else {
         // 2. low-precision LU
-        magma_int_t rc = dense_sgetrf(mode, n, n, c.S, c.lds, d_lpipiv, d_small + 1, dw, st);
+        StreamHolder side;
+        side.s = dense_side_stream(n, 4);
+        magma_int_t rc = dense_sgetrf(mode, n, n, c.S, c.lds, d_lpipiv, d_small + 1, dw, st, 0, side.s);
         if (rc != MAGMA_SUCCESS) return rc;
         if ((rc = dense_check_err(dw, st)) != MAGMA_SUCCESS) return rc;
         dn_finite_kernel<<<1024, kRT, 0, st>>>((size_t)n * n, n, c.S, c.lds, d_small + 2);
@@ -889,13 +910,13 @@ magma_int_t dense_iteref(int trans, int n, int nrhs, double* dA, int ldda, int* 
             // (src/dxgesv_gmres_gpu.cpp:211-219, 328-359, 574-575)
             const int itermax_all = std::min(n, 500);
             const int itermax = nested ? std::max(10, itermax_all / 50) : itermax_all;
-            GmBuf gin, gout;
+            GmOwner gin, gout;
             if (nested) MGI_HIP_CHECK(gin.alloc(n, std::min(n, kGmresInner)));
             if (gm_outer) MGI_HIP_CHECK(gout.alloc(n, itermax));
             int steps = 0;
             bool conv = false;
-            rc = gm_outer ? ir_fgmres(c, itermax, gout, nested ? &gin : nullptr, &steps, &conv)
-                          : ir_classical(c, itermax, nested ? &gin : nullptr, dz, &steps, &conv);
+            rc = gm_outer ? ir_fgmres(c, itermax, gout.g, nested ? &gin.g : nullptr, &steps, &conv)
+                          : ir_classical(c, itermax, nested ? &gin.g : nullptr, dz, &steps, &conv);
             if (rc != MAGMA_SUCCESS) return rc;
             if ((rc = dense_solve_check_err(c.dtrs ? c.swd : c.sw, st)) != MAGMA_SUCCESS) return rc;
             if (conv) it = steps;

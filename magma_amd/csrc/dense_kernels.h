@@ -35,6 +35,21 @@ struct DenseWork {
     void* Bh;        // FP32 storage: 16-bit U12 (columns, k contiguous), likewise
     size_t h_elems;  // capacity of each half
     int gmax;        // panel workgroups the buffers hold
+    // the same row-move, inverse and 16-bit operand buffers for the second
+    // stream of the look-ahead (the trailing update of the columns beyond
+    // the next block runs there while the next block is factored)
+    int* pairs2;
+    int* npairs2;
+    void* X2;
+    void* Ah2;
+    void* Bh2;
+    // this scratch as the second stream uses it
+    DenseWork side() const
+    {
+        DenseWork s = *this;
+        s.pairs = pairs2; s.npairs = npairs2; s.X = X2; s.Ah = Ah2; s.Bh = Bh2;
+        return s;
+    }
 };
 
 size_t dense_work_bytes(int m, int n, int elem, int gmax);
@@ -50,11 +65,15 @@ int device_cu_count();
 // run in the storage precision. Returns MAGMA_ERR_NOT_SUPPORTED when the
 // matrix has more rows than the panel kernel can hold on the device.
 // nb: block width (a multiple of the panel width, at most kDenseMaxNB), 0 for
-// the default.
+// the default. side: a second stream for the look-ahead (null: everything on
+// stream in order); the work on it is joined back into stream before return.
 magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipiv, int* d_info,
-                        const DenseWork& w, hipStream_t stream, int nb = 0);
+                        const DenseWork& w, hipStream_t stream, int nb = 0, hipStream_t side = nullptr);
 magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info,
-                         const DenseWork& w, hipStream_t stream, int nb = 0);
+                         const DenseWork& w, hipStream_t stream, int nb = 0, hipStream_t side = nullptr);
+// The look-ahead's second stream, or null when the order is too small to
+// gain. Owned by the caller (hipStreamDestroy).
+hipStream_t dense_side_stream(int m, int elem);
 // Reads the panel kernel's error word (synchronises the stream).
 magma_int_t dense_check_err(const DenseWork& w, hipStream_t stream);
 
@@ -65,7 +84,7 @@ struct SolveWork {
     void* invL;      // nblk x BS x BS inverses of the unit-lower diagonal blocks (T)
     void* invU;      // nblk x BS x BS inverses of the upper diagonal blocks (T)
     int* perm;       // n: row map of the interchanges (row i of P*B is B[perm[i]])
-    int* flags;      // [nchunk][nblk] block-ready flags of the triangular solves
+    unsigned long long* xw;   // tagged words of the solved values (2 per value, 8-column chunks)
     int* ticket;     // [0] start order of the solve workgroups
     int* err;        // [0] a wait exceeded its time limit
     void* Y;         // n x nrhs_max staging (T)

@@ -32,6 +32,9 @@
 #include "lu_device.h"
 #include "dense_kernels.h"
 
+#include <cstdlib>
+#include <vector>
+
 #include <algorithm>
 #include <climits>
 #include <type_traits>
@@ -904,10 +907,16 @@ size_t dense_work_bytes(int m, int n, int elem, int gmax)
     b += a256(sizeof(int) * 4);                         // err
     b += a256(sizeof(int) * 2 * 2 * NB);                // pairs
     b += a256(sizeof(int) * 4);                         // npairs
-    b += a256((size_t)elem * kDenseStrip * kDenseStrip);   // X
-    if (elem == 4) {                                          // Ah, Bh (hi and lo halves)
-        const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
-        b += 2 * a256(2 * 2 * h);
+    for (int lane = 0; lane < 2; ++lane) {                    // main stream, then the look-ahead's
+        if (lane == 1) {
+            b += a256(sizeof(int) * 2 * 2 * NB);            // pairs
+            b += a256(sizeof(int) * 4);                     // npairs
+        }
+        b += a256((size_t)elem * kDenseStrip * kDenseStrip);   // X
+        if (elem == 4) {                                          // Ah, Bh (hi and lo halves)
+            const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
+            b += 2 * a256(2 * 2 * h);
+        }
     }
     return b;
 }
@@ -923,13 +932,20 @@ void dense_work_carve(void* base, int m, int n, int elem, int gmax, DenseWork& w
     w.pairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 2 * 2 * NB);
     w.npairs = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 4);
     w.X = p;                             p += a256((size_t)elem * kDenseStrip * kDenseStrip);
-    w.Ah = w.Bh = nullptr;
+    w.Ah = w.Bh = w.Ah2 = w.Bh2 = nullptr;
     w.h_elems = 0;
+    const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
     if (elem == 4) {
-        const size_t h = (size_t)std::max(std::max(m, n), 1) * kDenseMaxNB;
         w.Ah = p;                        p += a256(2 * 2 * h);
-        w.Bh = p;
+        w.Bh = p;                        p += a256(2 * 2 * h);
         w.h_elems = h;
+    }
+    w.pairs2 = reinterpret_cast<int*>(p);  p += a256(sizeof(int) * 2 * 2 * NB);
+    w.npairs2 = reinterpret_cast<int*>(p); p += a256(sizeof(int) * 4);
+    w.X2 = p;                              p += a256((size_t)elem * kDenseStrip * kDenseStrip);
+    if (elem == 4) {
+        w.Ah2 = p;                       p += a256(2 * 2 * h);
+        w.Bh2 = p;
     }
     w.gmax = gmax;
 }
@@ -1091,14 +1107,55 @@ magma_int_t trsm_update(int mode, int m, int r0, int kw, int cb, int ce, T* A, i
     return rc;
 }
 
-// Blocks of nb columns (a multiple of the panel width PW), each factored as
-// nb / PW panels: after panel i, its interchanges go to the block's other
-// columns and its U rows and update to the block's columns on its right;
-// after the block, its interchanges go to every column outside it and its
-// U12 and trailing update (K = nb) to the columns on its right.
+// Factors block [k, k+jb) in place: nb / PW panels; after panel i, its
+// interchanges go to the block's other columns and its U rows and update to
+// the block's columns on its right.
+template <typename T>
+magma_int_t factor_block(int mode, int m, int k, int jb, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
+                         hipStream_t st)
+{
+    constexpr int PW = DenseShape<T>::PW;
+    magma_int_t rc;
+    for (int i0 = 0; i0 < jb; i0 += PW) {
+        const int wdt = std::min(PW, jb - i0);
+        const int kp = k + i0;
+        if ((rc = panel_launch<T>(m, kp, wdt, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+        const int right = kp + wdt;
+        if (i0 > 0 || right < k + jb) {
+            if ((rc = swaps_launch<T>(m, ipiv, kp, wdt, A, lda, w, k, kp, right, k + jb, st)) != MAGMA_SUCCESS)
+                return rc;
+        }
+        if (right < k + jb) {
+            if ((rc = trsm_update<T>(mode, m, kp, wdt, right, k + jb, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+        }
+    }
+    return MAGMA_SUCCESS;
+}
+
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~EventPair()
+    {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+// Blocks of nb columns (a multiple of the panel width PW). After block k is
+// factored, its interchanges go to every column outside it and its U12 and
+// trailing update (K = nb) to the columns on its right.
+//
+// With a second stream (look-ahead of one block): the stream in order
+// updates only the next block's columns and goes on to factor that block,
+// while the second stream applies the interchanges to the columns on the
+// left and updates every column beyond the next block. Before the first
+// stream updates the block after next, it waits for the second stream's
+// update of the previous step, which covered those columns. The two streams
+// touch disjoint columns between the events, and each has its own row-move,
+// inverse and 16-bit operand buffers.
 template <typename T>
 magma_int_t dense_getrf_t(int mode, int nb, int m, int n, T* A, int lda, int* ipiv, int* info, const DenseWork& w,
-                          hipStream_t st)
+                          hipStream_t st, hipStream_t side)
 {
     constexpr int PW = DenseShape<T>::PW;
     if (nb < PW || nb % PW != 0) return MAGMA_ERR_NOT_SUPPORTED;
@@ -1111,43 +1168,78 @@ magma_int_t dense_getrf_t(int mode, int nb, int m, int n, T* A, int lda, int* ip
     MGI_LAUNCH_CHECK();
     const int minmn = std::min(m, n);
     magma_int_t rc = MAGMA_SUCCESS;
+    if (side == nullptr || minmn <= nb) {
+        for (int k = 0; k < minmn; k += nb) {
+            const int jb = std::min(nb, minmn - k);
+            if ((rc = factor_block<T>(mode, m, k, jb, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w, 0, k, k + jb, n, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = trsm_update<T>(mode, m, k, jb, k + jb, n, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+        }
+        return MAGMA_SUCCESS;
+    }
+    const DenseWork w2 = w.side();
+    EventPair ev;     // e[0]: the first stream's block is ready; e[1]: the second stream's update is done
+    for (hipEvent_t& x : ev.e) MGI_HIP_CHECK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    // the second stream starts after what the caller queued on the first
+    MGI_HIP_CHECK(hipEventRecord(ev.e[0], st));
+    MGI_HIP_CHECK(hipStreamWaitEvent(side, ev.e[0], 0));
+    bool side_pending = false;
     for (int k = 0; k < minmn; k += nb) {
         const int jb = std::min(nb, minmn - k);
-        for (int i0 = 0; i0 < jb; i0 += PW) {
-            const int wdt = std::min(PW, jb - i0);
-            const int kp = k + i0;
-            if ((rc = panel_launch<T>(m, kp, wdt, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
-            const int right = kp + wdt;
-            if (i0 > 0 || right < k + jb) {
-                if ((rc = swaps_launch<T>(m, ipiv, kp, wdt, A, lda, w, k, kp, right, k + jb, st)) != MAGMA_SUCCESS)
-                    return rc;
-            }
-            if (right < k + jb) {
-                if ((rc = trsm_update<T>(mode, m, kp, wdt, right, k + jb, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
-            }
+        if ((rc = factor_block<T>(mode, m, k, jb, A, lda, ipiv, info, w, st)) != MAGMA_SUCCESS) return rc;
+        const int c0 = k + jb;                       // next block [c0, c1), beyond it [c1, n)
+        const int c1 = std::min(n, c0 + nb);
+        // the next block's columns were last written by the second stream's
+        // update of the previous step
+        if (side_pending) MGI_HIP_CHECK(hipStreamWaitEvent(st, ev.e[1], 0));
+        if (c1 > c0) {
+            if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w, c0, c1, 0, 0, st)) != MAGMA_SUCCESS) return rc;
+            if ((rc = trsm_update<T>(mode, m, k, jb, c0, c1, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
         }
-        if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w, 0, k, k + jb, n, st)) != MAGMA_SUCCESS) return rc;
-        if ((rc = trsm_update<T>(mode, m, k, jb, k + jb, n, A, lda, w, st)) != MAGMA_SUCCESS) return rc;
+        MGI_HIP_CHECK(hipEventRecord(ev.e[0], st));
+        MGI_HIP_CHECK(hipStreamWaitEvent(side, ev.e[0], 0));
+        if ((rc = swaps_launch<T>(m, ipiv, k, jb, A, lda, w2, 0, k, c1, n, side)) != MAGMA_SUCCESS) return rc;
+        if (c1 < n && (rc = trsm_update<T>(mode, m, k, jb, c1, n, A, lda, w2, side)) != MAGMA_SUCCESS) return rc;
+        MGI_HIP_CHECK(hipEventRecord(ev.e[1], side));
+        side_pending = true;
     }
+    MGI_HIP_CHECK(hipStreamWaitEvent(st, ev.e[1], 0));
     return MAGMA_SUCCESS;
 }
 
 }  // namespace
 
 magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
-                        hipStream_t stream, int nb)
+                        hipStream_t stream, int nb, hipStream_t side)
 {
     (void)mode;
     return dense_getrf_t<double>(kUpdF64, nb > 0 ? nb : DenseShape<double>::NB, m, n, dA, ldda, d_ipiv, d_info, w,
-                                 stream);
+                                 stream, side);
 }
 
 magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
-                         hipStream_t stream, int nb)
+                         hipStream_t stream, int nb, hipStream_t side)
 {
     if (mode != kUpdF32 && mode != kUpdBF16 && mode != kUpdFP16 && mode != kUpdBF16x3)
         return MAGMA_ERR_NOT_SUPPORTED;
-    return dense_getrf_t<float>(mode, nb > 0 ? nb : DenseShape<float>::NB, m, n, dA, ldda, d_ipiv, d_info, w, stream);
+    return dense_getrf_t<float>(mode, nb > 0 ? nb : DenseShape<float>::NB, m, n, dA, ldda, d_ipiv, d_info, w, stream,
+                                side);
+}
+
+hipStream_t dense_side_stream(int m, int elem)
+{
+    // below this order a block step is too short for two streams to gain;
+    // MAGMA_DENSE_LOOKAHEAD_MIN overrides the order (tests run the two-stream
+    // schedule at small orders with it)
+    int min_m = 4096;
+    if (const char* e = std::getenv("MAGMA_DENSE_LOOKAHEAD_MIN")) min_m = std::atoi(e);
+    if (m < min_m) return nullptr;
+    (void)elem;
+    // an ordinary stream: a stream with a CU mask needs a hardware queue of
+    // its own, and a process has few (creating them per call fails)
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    return s;
 }
 
 magma_int_t dense_check_err(const DenseWork& w, hipStream_t stream)

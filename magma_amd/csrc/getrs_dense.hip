@@ -8,10 +8,14 @@
 // launch: the rows are cut into blocks of 128; the workgroup that takes
 // block b (in the order the workgroups start, through a ticket counter)
 // subtracts the products of the off-diagonal blocks with every solved block
-// as soon as that block's flag is up, then multiplies by the precomputed
-// inverse of its diagonal block, stores its rows and raises its own flag.
-// A workgroup only waits for blocks taken by workgroups that started
-// before it, so the solve cannot deadlock whatever the residency.
+// as soon as that block's values are published, then multiplies by the
+// precomputed inverse of its diagonal block and publishes its own rows.
+// A solved value is published as tagged 64-bit words ((epoch << 32) |
+// 32 bits of the value; two words for FP64), so a reader that sees the
+// solve's epoch in every word it needs has the values themselves: one
+// memory round trip per block, no flag and no fence. A workgroup only waits
+// for blocks taken by workgroups that started before it, so the solve cannot
+// deadlock whatever the residency.
 #include "magma_internal.h"
 #include "lu_device.h"
 #include "dense_kernels.h"
@@ -35,6 +39,54 @@ __device__ __forceinline__ int ld_agent_i(const int* p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The low 32 bits of the word at p once its high half is epoch (bounded by
+// kSolveWaitTicks from t0; a timeout, or another workgroup's, raises err and
+// returns 0).
+__device__ __forceinline__ unsigned wait_word(const unsigned long long* p, unsigned epoch, int* err,
+                                              unsigned long long t0)
+{
+    for (;;) {
+        const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((unsigned)(v >> 32) == epoch) return (unsigned)v;
+        if (ld_agent_i(err) != 0) return 0u;
+        if ((long long)(__builtin_amdgcn_s_memrealtime() - t0) > kSolveWaitTicks) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return 0u;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ void put_word(unsigned long long* p, unsigned epoch, unsigned payload)
+{
+    __hip_atomic_store(p, ((unsigned long long)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T> struct XWords;
+template <> struct XWords<float> {
+    static constexpr int W = 1;
+    __device__ static float get(const unsigned long long* p, unsigned e, int* err, unsigned long long t0)
+    {
+        return __uint_as_float(wait_word(p, e, err, t0));
+    }
+    __device__ static void put(unsigned long long* p, unsigned e, float v) { put_word(p, e, __float_as_uint(v)); }
+};
+template <> struct XWords<double> {
+    static constexpr int W = 2;
+    __device__ static double get(const unsigned long long* p, unsigned e, int* err, unsigned long long t0)
+    {
+        const unsigned lo = wait_word(p, e, err, t0);
+        const unsigned hi = wait_word(p + 1, e, err, t0);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+    __device__ static void put(unsigned long long* p, unsigned e, double v)
+    {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        put_word(p, e, (unsigned)b);
+        put_word(p + 1, e, (unsigned)(b >> 32));
+    }
+};
+
 }  // namespace
 
 // KIND 0: L (unit lower), forward; 1: U, backward; 2: U^T (lower), forward;
@@ -43,12 +95,14 @@ __device__ __forceinline__ int ld_agent_i(const int* p)
 // columns of each off-diagonal block. Its 64 matrix values of a block do not
 // depend on the solution, so they are loaded before the block's flag is
 // awaited (one memory latency per block instead of one per column).
+// xw: [nchunk][nblk][NR][BS][W] tagged words of the solved values.
 template <typename T, int KIND, int NR>
 __global__ void __launch_bounds__(256)
 dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __restrict__ invD,
-                  T* __restrict__ X, int ldx, int* __restrict__ flags, int* __restrict__ ticket,
-                  int* __restrict__ err, int epoch)
+                  T* __restrict__ X, int ldx, unsigned long long* __restrict__ xw, int* __restrict__ ticket,
+                  int* __restrict__ err, unsigned epoch)
 {
+    constexpr int W = XWords<T>::W;
     constexpr bool kFwd = (KIND == 0 || KIND == 2);
     constexpr bool kTr = (KIND == 2 || KIND == 3);
     constexpr int H = BS / 2;
@@ -68,7 +122,7 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
     const int nb = min(BS, n - r0);
     const int q0 = chunk * NR;
     const int nq = min(NR, nrhs - q0);
-    int* fl = flags + (size_t)chunk * nblk;
+    unsigned long long* xwc = xw + (size_t)chunk * nblk * (NR * BS * W);
 
     T acc[NR];
 #pragma unroll
@@ -89,22 +143,13 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
                                                   : LU[(size_t)(r0 + rr) + (size_t)(c0 + col) * ld])
                                            : T(0);
         }
-        if (t == 0) {
-            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-            while (ld_agent_i(&fl[c]) != epoch) {
-                if (ld_agent_i(err) != 0) break;
-                if ((long long)(__builtin_amdgcn_s_memrealtime() - t0) > kSolveWaitTicks) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        }
-        __syncthreads();
+        // the values of block c, published by its workgroup
+        __syncthreads();      // xs of the previous block consumed
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        const unsigned long long* xc = xwc + (size_t)c * (NR * BS * W);
         for (int e = t; e < BS * NR; e += 256) {
             const int i = e % BS, q = e / BS;
-            xs[i][q] = (i < cn && q < nq) ? X[(size_t)(c0 + i) + (size_t)(q0 + q) * ldx] : T(0);
+            xs[i][q] = (i < cn && q < nq) ? XWords<T>::get(xc + (size_t)(q * BS + i) * W, epoch, err, t0) : T(0);
         }
         __syncthreads();
 #pragma unroll
@@ -147,13 +192,15 @@ dense_trsv_kernel(int n, int nrhs, const T* __restrict__ LU, int ld, const T* __
     }
     __syncthreads();
     if (h == 0 && rr < nb) {
+        unsigned long long* xb = xwc + (size_t)b * (NR * BS * W);
 #pragma unroll
         for (int q = 0; q < NR; ++q)
-            if (q < nq) X[(size_t)(r0 + rr) + (size_t)(q0 + q) * ldx] = out[q] + red[rr][q];
+            if (q < nq) {
+                const T v = out[q] + red[rr][q];
+                XWords<T>::put(xb + (size_t)(q * BS + rr) * W, epoch, v);
+                X[(size_t)(r0 + rr) + (size_t)(q0 + q) * ldx] = v;
+            }
     }
-    __threadfence();
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(&fl[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Y (ldy) = rows perm[i] of S (NoTrans gather), or X[perm[i]] = Y[i] (scatter),
@@ -176,11 +223,11 @@ __global__ void dense_scatter_kernel(int n, int nrhs, const int* __restrict__ pe
     if (i < n && q < nrhs) D[(size_t)(perm ? perm[i] : i) + (size_t)q * ldd] = (TD)S[(size_t)i + (size_t)q * lds];
 }
 
-__global__ void dense_solve_reset_kernel(int* flags, int nflags, int* ticket, int* err)
+__global__ void dense_solve_reset_kernel(unsigned long long* xw, size_t nwords, int* ticket, int* err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nflags) flags[i] = 0;
-    if (i == 0) { *ticket = 0; *err = 0; }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+        xw[i] = 0ull;                       // epoch 0 is never a solve's
+    if (blockIdx.x == 0 && threadIdx.x == 0) { *ticket = 0; *err = 0; }
 }
 
 __global__ void dense_ticket_reset_kernel(int* ticket) { *ticket = 0; }
@@ -197,7 +244,7 @@ size_t solve_work_bytes(int n, int nrhs_max, int elem)
     size_t b = 0;
     b += 2 * a256s((size_t)elem * nblk * BS * BS);        // invL, invU
     b += a256s(sizeof(int) * std::max(n, 1));             // perm
-    b += a256s(sizeof(int) * (size_t)nblk * nchunk);      // flags
+    b += a256s(sizeof(unsigned long long) * 2 * (size_t)nblk * BS * nchunk * 8);   // tagged values
     b += a256s(sizeof(int) * 4);                          // ticket
     b += a256s(sizeof(int) * 4);                          // err
     b += a256s((size_t)elem * std::max(n, 1) * std::max(nrhs_max, 1));   // Y
@@ -212,7 +259,8 @@ void solve_work_carve(void* base, int n, int nrhs_max, int elem, SolveWork& s)
     s.invL = p;                            p += a256s((size_t)elem * nblk * BS * BS);
     s.invU = p;                            p += a256s((size_t)elem * nblk * BS * BS);
     s.perm = reinterpret_cast<int*>(p);    p += a256s(sizeof(int) * std::max(n, 1));
-    s.flags = reinterpret_cast<int*>(p);   p += a256s(sizeof(int) * (size_t)nblk * nchunk);
+    s.xw = reinterpret_cast<unsigned long long*>(p);
+    p += a256s(sizeof(unsigned long long) * 2 * (size_t)nblk * BS * nchunk * 8);
     s.ticket = reinterpret_cast<int*>(p);  p += a256s(sizeof(int) * 4);
     s.err = reinterpret_cast<int*>(p);     p += a256s(sizeof(int) * 4);
     s.Y = p;
@@ -255,19 +303,23 @@ magma_int_t trsv_pair(int trans, int n, int nrhs, const T* LU, int ld, T* Y, int
     const int nwg = nblk * nchunk;
     const T* iL = reinterpret_cast<const T*>(s.invL);
     const T* iU = reinterpret_cast<const T*>(s.invU);
-    dense_solve_reset_kernel<<<(unsigned)ceildiv(nwg, 256), 256, 0, st>>>(s.flags, nwg, s.ticket, s.err);
+    // the tagged words this pair uses, cleared (capacity: nrhs_max <= 64)
+    const size_t nwords = (size_t)nchunk * nblk * NR * BS * XWords<T>::W;
+    if (nwords > (size_t)2 * nblk * BS * ceildiv(s.nrhs_max, 8) * 8) return MAGMA_ERR_NOT_SUPPORTED;
+    dense_solve_reset_kernel<<<(unsigned)std::min<size_t>(ceildiv((long)nwords, 256L), 1024), 256, 0, st>>>(
+        s.xw, nwords, s.ticket, s.err);
     MGI_LAUNCH_CHECK();
-    const int e1 = 1, e2 = 2;     // the flags are cleared for every pair of solves
+    const unsigned e1 = 1, e2 = 2;     // the words are cleared for every pair of solves
     if (trans == 0) {
-        dense_trsv_kernel<T, 0, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.flags, s.ticket, s.err, e1);
+        dense_trsv_kernel<T, 0, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.xw, s.ticket, s.err, e1);
         MGI_LAUNCH_CHECK();
         dense_ticket_reset_kernel<<<1, 1, 0, st>>>(s.ticket);
-        dense_trsv_kernel<T, 1, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.flags, s.ticket, s.err, e2);
+        dense_trsv_kernel<T, 1, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.xw, s.ticket, s.err, e2);
     } else {
-        dense_trsv_kernel<T, 2, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.flags, s.ticket, s.err, e1);
+        dense_trsv_kernel<T, 2, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iU, Y, ldy, s.xw, s.ticket, s.err, e1);
         MGI_LAUNCH_CHECK();
         dense_ticket_reset_kernel<<<1, 1, 0, st>>>(s.ticket);
-        dense_trsv_kernel<T, 3, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.flags, s.ticket, s.err, e2);
+        dense_trsv_kernel<T, 3, NR><<<nwg, 256, 0, st>>>(n, nrhs, LU, ld, iL, Y, ldy, s.xw, s.ticket, s.err, e2);
     }
     MGI_LAUNCH_CHECK();
     return MAGMA_SUCCESS;

@@ -156,6 +156,10 @@ inline double __hiloint2double(int hi, int lo)
 }
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
+inline float __uint_as_float(unsigned i) { float f; std::memcpy(&f, &i, 4); return f; }
+inline unsigned __float_as_uint(float f) { unsigned i; std::memcpy(&i, &f, 4); return i; }
+inline double __longlong_as_double(long long i) { double f; std::memcpy(&f, &i, 8); return f; }
+inline long long __double_as_longlong(double f) { long long i; std::memcpy(&i, &f, 8); return i; }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicCAS(int* p, int cmp, int v)
@@ -292,6 +296,14 @@ inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_ca
 inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (void*)0x1; return 0; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)0x1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
+// launches run when issued, so every stream is the one host order, which
+// respects any event dependencies the host expressed
+inline hipError_t hipExtStreamCreateWithCUMask(hipStream_t* s, uint32_t, const uint32_t*) { *s = (void*)0x2; return 0; }
+#define hipEventDisableTiming 2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = (void*)0x3; return 0; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t = nullptr) { return 0; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return 0; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return 0; }
 inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return 1; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return 1; }
 inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) { return 1; }

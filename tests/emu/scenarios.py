@@ -31,7 +31,8 @@ libc.free.argtypes = [ctypes.c_void_p]
 vp, i32 = ctypes.c_void_p, ctypes.c_int
 
 PREC = {"fp32": 900, "fp16": 905, "bf16": 940, "bf16x3": 941}
-SOLV = {"ir": 920, "gmres": 924}
+SOLV = {"ir": 920, "gmres": 924, "irstrs": 920, "irdtrs": 921, "irgmstrs": 922, "irgmdtrs": 923, "gmstrs": 924,
+        "gmdtrs": 925, "gmgmstrs": 926, "gmgmdtrs": 927}
 
 
 class Buf:

@@ -71,6 +71,29 @@ def test_dense_getrf_4096(dn, oracle):
     assert oracle.lu_residual(A, LU[None], ipiv[None], n, n)[0] < TESTER_TOL
 
 
+@pytest.mark.parametrize("prec,m,n", [("d", 2049, 2049), ("d", 1800, 2300), ("s", 2600, 2600), ("bf16", 2100, 1900)])
+def test_dense_lookahead_matches_serial(dn, monkeypatch, prec, m, n):
+    """The two-stream schedule (the next block factored while the rest of the
+    trailing matrix is updated) gives bit-identical factors and pivots to the
+    one-stream schedule: every column receives the same updates in the same
+    order. Forced at these orders with MAGMA_DENSE_LOOKAHEAD_MIN."""
+    rng = np.random.default_rng(m + n)
+    Arow = rng.uniform(-1, 1, (m, n))
+    dt = torch.float64 if prec == "d" else torch.float32
+    out = []
+    for lam in ("1000000000", "0"):
+        monkeypatch.setenv("MAGMA_DENSE_LOOKAHEAD_MIN", lam)
+        dA = torch.from_numpy(cm(Arow)).to(dt).cuda()
+        if prec == "d":
+            ipiv, info = dn.getrf_gpu(dA, m=m)
+        else:
+            ipiv, info = dn.sgetrf_prec_gpu(dA, "fp32" if prec == "s" else prec, m=m)
+        assert info == 0
+        out.append((dA.cpu().numpy(), np.array(ipiv)))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_dense_native_matches_gpu(dn):
     rng = np.random.default_rng(3)
     Arow = rng.uniform(-1, 1, (777, 777))
