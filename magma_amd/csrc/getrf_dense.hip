@@ -138,6 +138,9 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
     T* P = A + k + (size_t)k * lda;
     const int gmax = w.gmax;
     unsigned long long* words = reinterpret_cast<unsigned long long*>(w.crow);
+    // the panel is the critical path of the factorisation: its waves go
+    // first where they share a SIMD with the trailing update's
+    __builtin_amdgcn_s_setprio(3);
 
     T a[PW];
 #pragma unroll
@@ -196,12 +199,14 @@ dense_panel_kernel(int m, int k, int jb, T* __restrict__ A, int lda, int* __rest
                     cg = 0;
                     for (int q = ln; q < G; q += kWave) {
                         const unsigned long long* sl = words + ((size_t)par * gmax + q) * SW;
+                        // header and value together (one round trip): each
+                        // word carries its own tag, a stale one is discarded
                         const unsigned long long hw = ld_word(sl);
+                        const unsigned long long w0 = ld_word(sl + 1 + j * NW);
+                        const unsigned long long w1 = NW == 2 ? ld_word(sl + 1 + j * NW + 1) : w0;
                         if ((unsigned)(hw >> 32) != tag) { ok = false; continue; }
                         const int p = (int)(unsigned)hw;
                         if (p < 0) continue;                        // no candidate in that workgroup
-                        const unsigned long long w0 = ld_word(sl + 1 + j * NW);
-                        const unsigned long long w1 = NW == 2 ? ld_word(sl + 1 + j * NW + 1) : w0;
                         if ((unsigned)(w0 >> 32) != tag || (unsigned)(w1 >> 32) != tag) { ok = false; continue; }
                         T x = fabs(value_of<T>((unsigned)w0, (unsigned)w1));
                         if (x != x) x = (p == diag) ? T(INFINITY) : T(-1);

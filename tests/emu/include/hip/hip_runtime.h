@@ -175,6 +175,7 @@ inline int atomicCAS(int* p, int cmp, int v)
 inline void __threadfence() {}
 // a wait yields to the other lanes and workgroups in flight
 inline void __builtin_amdgcn_s_sleep(int) { emu::yield(); }
+inline void __builtin_amdgcn_s_setprio(int) {}
 inline unsigned long long __builtin_amdgcn_s_memrealtime() { return emu::realtime(); }
 inline int atomicSub(int* p, int v) { return __atomic_fetch_sub(p, v, __ATOMIC_SEQ_CST); }
 
