@@ -401,6 +401,20 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs,
         magma_refinement_t facto_type, magma_refinement_t solver_type, magma_int_t itermax,
         magma_int_t *iter_array, magma_int_t *info_array, magma_int_t batchCount, magma_queue_t queue);
 
+/* ------------------------------------------------------------------------- */
+/* Adjoint of the batched solve (differentiable solve)                        */
+/* ------------------------------------------------------------------------- */
+/* No MAGMA counterpart: the backward pass of X = A^{-1} B needs
+ * grad_A = -grad_B X^T. For contiguous batches of column-major n x nrhs
+ * grad_B and X (leading dimensions ldb, ldx; strideB, strideX elements
+ * between systems) it writes the row-major n x n grad_A (row stride lda,
+ * system stride strideA). elem: 8 for FP64, 4 for FP32. Asynchronous on the
+ * queue. */
+magma_int_t magma_adjoint_outer_batched(magma_int_t elem, magma_int_t n, magma_int_t nrhs, const void *dgB,
+                                        magma_int_t ldb, long long strideB, const void *dX, magma_int_t ldx,
+                                        long long strideX, void *dgA, magma_int_t lda, long long strideA,
+                                        magma_int_t batchCount, magma_queue_t queue);
+
 /* No MAGMA counterpart. For callers that cannot allocate device memory (code
  * under hipGraph capture, tight serving loops, harnesses that forbid
  * hipMalloc): the LU of magma_dgetrf_native (same kernels, same results),

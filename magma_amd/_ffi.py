@@ -97,6 +97,8 @@ _PROTOS = {
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
     "magma_dbgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
+    "magma_adjoint_outer_batched": (c_int, [c_int, c_int, c_int, c_void_p, c_int, ctypes.c_longlong, c_void_p, c_int,
+                                            ctypes.c_longlong, c_void_p, c_int, ctypes.c_longlong, c_int, c_void_p]),
     "magma_dsgesv_iteref_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,
                                         c_int, c_void_p, c_int, c_void_p, c_void_p, i32p, i32p]),
     "magma_dxgesv_gmres_gpu": (c_int, [c_int, c_int, c_int, c_void_p, c_int, i32p, c_void_p, c_void_p,

@@ -3,7 +3,8 @@
 Forward: one batched LU of A (partial pivoting, LAPACK semantics) and one
 batched triangular solve. Backward, for a loss with dL/dX = G:
     grad_B = A^{-T} G            (a transposed solve with the stored LU + ipiv)
-    grad_A = -grad_B X^T         (a rank-nrhs outer product per system)
+    grad_A = -grad_B X^T         (a rank-nrhs outer product per system,
+                                  magma_adjoint_outer_batched)
 No second factorisation is done: the factors saved by the forward pass are
 reused (the adjoint of src/zgesv_batched.cpp's getrf + getrs).
 
@@ -18,7 +19,7 @@ from __future__ import annotations
 import torch
 
 from . import _ffi
-from .batched import dgetrf_batched, dgetrs_batched, sgetrf_batched, sgetrs_batched
+from .batched import MagmaError, default_queue, dgetrf_batched, dgetrs_batched, sgetrf_batched, sgetrs_batched
 
 
 def _getrf(LU: torch.Tensor):
@@ -31,6 +32,20 @@ def _getrf(LU: torch.Tensor):
 
 def _getrs(trans: int, LU: torch.Tensor, ipiv: torch.Tensor, B_cm: torch.Tensor) -> None:
     (dgetrs_batched if LU.dtype == torch.float64 else sgetrs_batched)(trans, LU, ipiv, B_cm)
+
+
+def _outer(gB_cm: torch.Tensor, X_cm: torch.Tensor) -> torch.Tensor:
+    """-grad_B X^T (row-major (batch, n, n)) from the column-major (batch,
+    nrhs, n) grad_B and X, on the library's kernel."""
+    batch, nrhs, n = gB_cm.shape
+    gA = torch.empty((batch, n, n), dtype=gB_cm.dtype, device=gB_cm.device)
+    q = default_queue(gB_cm.device)
+    rc = _ffi.load().magma_adjoint_outer_batched(gB_cm.element_size(), n, nrhs, gB_cm.data_ptr(), n, nrhs * n,
+                                                 X_cm.data_ptr(), n, nrhs * n, gA.data_ptr(), n, n * n, batch,
+                                                 q.handle)
+    if rc < 0:
+        raise MagmaError("magma_adjoint_outer_batched", rc)
+    return gA
 
 
 def _to_cm(M: torch.Tensor) -> torch.Tensor:
@@ -52,19 +67,17 @@ class BatchedSolve(torch.autograd.Function):
         X_cm = _to_cm(B.detach())                      # (batch, nrhs, n)
         _getrs(_ffi.MagmaTrans, LU, ipiv, X_cm)        # (A^T)^T X = B
         X = X_cm.transpose(1, 2)
-        ctx.save_for_backward(LU, ipiv, X)
+        ctx.save_for_backward(LU, ipiv, X_cm)
         ctx.mark_non_differentiable(info)
         return X, info
 
     @staticmethod
     def backward(ctx, G: torch.Tensor, _ginfo):
-        LU, ipiv, X = ctx.saved_tensors
+        LU, ipiv, X_cm = ctx.saved_tensors
         gB_cm = _to_cm(G)
         _getrs(_ffi.MagmaNoTrans, LU, ipiv, gB_cm)     # A^T gB = G
         gB = gB_cm.transpose(1, 2)
-        gA = None
-        if ctx.needs_input_grad[0]:
-            gA = -torch.bmm(gB.contiguous(), X.transpose(1, 2).contiguous())
+        gA = _outer(gB_cm, X_cm) if ctx.needs_input_grad[0] else None
         return gA, (gB if ctx.needs_input_grad[1] else None)
 
 
