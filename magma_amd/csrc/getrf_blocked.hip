@@ -339,6 +339,10 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
 //    accumulators (4 per lane, r = 0..3):
 //      f64:  C[4*r + l/16][l%16]
 //      f32:  C[4*(l/16) + r][l%16]
+//    The product is formed transposed (A12^T as the first operand, A21^T as
+//    the second), so lane l holds row l%16 of the tile: each store writes 16
+//    consecutive rows of one column (one contiguous segment per column)
+//    instead of one element in each of 16 columns.
 // ---------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -402,19 +406,19 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
                 for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                     for (int tj = 0; tj < 2; ++tj)
-                        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+                        acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[tj], af[ti], acc[ti][tj], 0, 0, 0);
             }
         }
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj) {
-                const int col = c0 + wc + 16 * tj + lr;
-                if (col >= nc) continue;
+                const int row = r0 + wr + 16 * ti + lr;
+                if (row >= mr) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = r0 + wr + 16 * ti + 4 * r + lq;
-                    if (row < mr) {
+                    const int col = c0 + wc + 16 * tj + 4 * r + lq;
+                    if (col < nc) {
                         T* cp = C + row + (size_t)col * ld;
                         *cp = *cp - acc[ti][tj][r];
                     }
@@ -439,7 +443,7 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
                     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                         for (int tj = 0; tj < 2; ++tj)
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[ti], bf[tj], acc[ti][tj], 0, 0, 0);
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[tj], af[ti], acc[ti][tj], 0, 0, 0);
                 }
             }
         } else {
@@ -468,7 +472,7 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
                     for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                         for (int tj = 0; tj < 2; ++tj)
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bh[tj], ah[ti], acc[ti][tj], 0, 0, 0);
                 } else {
                     bf16x8_t ah[2], bh[2], al[2], bl[2];
 #pragma unroll
@@ -486,10 +490,10 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
 #pragma unroll
                         for (int tj = 0; tj < 2; ++tj) {
                             if constexpr (MODE == kUpdBF16x3) {
-                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[ti], bh[tj], acc[ti][tj], 0, 0, 0);
-                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bl[tj], acc[ti][tj], 0, 0, 0);
+                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[tj], al[ti], acc[ti][tj], 0, 0, 0);
+                                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[tj], ah[ti], acc[ti][tj], 0, 0, 0);
                             }
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[ti], bh[tj], acc[ti][tj], 0, 0, 0);
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[tj], ah[ti], acc[ti][tj], 0, 0, 0);
                         }
                 }
             }
@@ -498,12 +502,12 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj) {
-                const int col = c0 + wc + 16 * tj + lr;
-                if (col >= nc) continue;
+                const int row = r0 + wr + 16 * ti + lr;
+                if (row >= mr) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = r0 + wr + 16 * ti + 4 * lq + r;
-                    if (row < mr) {
+                    const int col = c0 + wc + 16 * tj + 4 * lq + r;
+                    if (col < nc) {
                         T* cp = C + row + (size_t)col * ld;
                         *cp = *cp - acc[ti][tj][r];
                     }
