@@ -64,6 +64,8 @@ typedef enum {
     Magma_PREC_XSH_163232TC = 917,
     Magma_PREC_SBT        = 940,   /* new: FP32 LU, BF16-input GEMM, FP32 accum.   */
     Magma_PREC_SB3T       = 941,   /* new: FP32 LU, 3-term BF16 split GEMM         */
+    Magma_PREC_SF8T       = 942,   /* new: FP32 LU, FP8 (e4m3) GEMM, FP32 accum.;
+                                      an experiment, one-system drivers only       */
     Magma_REFINE_IRSTRS   = 920,   /* classical IR, low-precision triangular solve */
     Magma_REFINE_IRDTRS   = 921,
     Magma_REFINE_IRGMSTRS = 922,

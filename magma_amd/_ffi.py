@@ -23,7 +23,7 @@ i32p = ctypes.POINTER(ctypes.c_int)
 
 MagmaNoTrans, MagmaTrans, MagmaConjTrans = 111, 112, 113
 Magma_PREC_SS, Magma_PREC_SHT, Magma_PREC_XSH_163232TC = 900, 905, 917
-Magma_PREC_SBT, Magma_PREC_SB3T = 940, 941
+Magma_PREC_SBT, Magma_PREC_SB3T, Magma_PREC_SF8T = 940, 941, 942
 (Magma_REFINE_IRSTRS, Magma_REFINE_IRDTRS, Magma_REFINE_IRGMSTRS, Magma_REFINE_IRGMDTRS, Magma_REFINE_GMSTRS,
  Magma_REFINE_GMDTRS, Magma_REFINE_GMGMSTRS, Magma_REFINE_GMGMDTRS) = range(920, 928)
 Magma_MP_ENABLE_DFLT_MATH, Magma_MP_ENABLE_TC_MATH = 960, 961

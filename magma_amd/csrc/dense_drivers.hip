@@ -1003,7 +1003,7 @@ magma_int_t magma_xshgetrf_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr dA, 
 
 // No MAGMA counterpart: the FP32 LU with the trailing-update format named by
 // a refinement factorisation type (Magma_PREC_SS FP32, _SHT FP16, _SBT BF16,
-// _SB3T split BF16), as used inside magma_dxgesv_gmres_gpu.
+// _SB3T split BF16, _SF8T FP8 e4m3), as used inside magma_dxgesv_gmres_gpu.
 magma_int_t magma_sgetrf_prec_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr dA, magma_int_t ldda,
                                   magma_int_t* ipiv, magma_int_t* info, magma_refinement_t facto_type)
 {
@@ -1013,6 +1013,7 @@ magma_int_t magma_sgetrf_prec_gpu(magma_int_t m, magma_int_t n, magmaFloat_ptr d
         case Magma_PREC_SHT: case Magma_PREC_XSH_163232TC: mode = kUpdFP16; break;
         case Magma_PREC_SBT: mode = kUpdBF16; break;
         case Magma_PREC_SB3T: mode = kUpdBF16x3; break;
+        case Magma_PREC_SF8T: mode = kUpdFP8; break;
         default: break;
     }
     if (mode < 0) {

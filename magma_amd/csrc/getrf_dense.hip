@@ -19,10 +19,11 @@
 //   U12      L11^{-1} (dense_trtri_kernel) times A12 on MFMA, in the storage
 //            precision (FP64, or FP32 as MAGMA's strsm).
 //   update   A22 -= A21 U12 on MFMA, 128 x 128 tiles: FP64 or FP32
-//            (dense_gemm_kernel); or FP32 storage with BF16, FP16 or
-//            split-BF16 inputs and FP32 accumulation, A21 and U12 converted
-//            once per update to 16-bit rows / columns with k contiguous
-//            (dense_cvt_a/b_kernel, dense_gemm_h_kernel).
+//            (dense_gemm_kernel); or FP32 storage with BF16, FP16,
+//            split-BF16 or (as an experiment) FP8 e4m3 inputs and FP32
+//            accumulation, A21 and U12 converted once per update to 16- or
+//            8-bit rows / columns with k contiguous (dense_cvt_a/b_kernel,
+//            dense_gemm_h_kernel).
 // Inside a block, each panel's interchanges go to the block's other columns
 // and its U rows and update to the block's columns on its right before the
 // next panel; the whole block's interchanges, U12 and update (K = nb) are
@@ -700,6 +701,22 @@ dense_gemm_kernel(int M, int N, int K, const T* __restrict__ A, int lda, const T
 // for k < ldk (K rounded up to 64; the padding is zero). With a split
 // (BF16x3) the second array of each holds the remainders lo = h(x - hi).
 // ---------------------------------------------------------------------------
+// Rounding to the update format and back: BF16 / FP16 by the conversions of
+// the types; FP8 (OCP e4m3, one byte) by v_cvt_pk_fp8_f32 (round to nearest
+// even, no scaling: multipliers are at most 1 in magnitude, and a U12 entry
+// beyond 448 becomes NaN, which the non-finite-factor check reports).
+template <typename h_t> struct HConv {
+    __device__ static h_t to(float x) { return (h_t)x; }
+    __device__ static float back(h_t v) { return (float)v; }
+};
+template <> struct HConv<unsigned char> {
+    __device__ static unsigned char to(float x)
+    {
+        return (unsigned char)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false) & 0xff);
+    }
+    __device__ static float back(unsigned char v) { return __builtin_amdgcn_cvt_f32_fp8((int)v, 0); }
+};
+
 template <typename h_t>
 __global__ void __launch_bounds__(256)
 dense_cvt_a_kernel(int M, int K, int ldk, const float* __restrict__ A, int lda, h_t* __restrict__ hi,
@@ -721,8 +738,8 @@ dense_cvt_a_kernel(int M, int K, int ldk, const float* __restrict__ A, int lda, 
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
             const float x = tile[8 * g + v][i];
-            vh[v] = (h_t)x;
-            vl[v] = (h_t)(x - (float)vh[v]);
+            vh[v] = HConv<h_t>::to(x);
+            vl[v] = HConv<h_t>::to(x - HConv<h_t>::back(vh[v]));
         }
         *reinterpret_cast<v8h*>(hi + (size_t)(i0 + i) * ldk + k0 + 8 * g) = vh;
         if (lo) *reinterpret_cast<v8h*>(lo + (size_t)(i0 + i) * ldk + k0 + 8 * g) = vl;
@@ -744,8 +761,8 @@ dense_cvt_b_kernel(int N, int K, int ldk, const float* __restrict__ B, int ldb, 
     for (int v = 0; v < 8; ++v) {
         const int kk = 8 * g + v;
         const float x = kk < K ? B[(size_t)kk + (size_t)j * ldb] : 0.f;
-        vh[v] = (h_t)x;
-        vl[v] = (h_t)(x - (float)vh[v]);
+        vh[v] = HConv<h_t>::to(x);
+        vl[v] = HConv<h_t>::to(x - HConv<h_t>::back(vh[v]));
     }
     *reinterpret_cast<v8h*>(hi + (size_t)j * ldk + 8 * g) = vh;
     if (lo) *reinterpret_cast<v8h*>(lo + (size_t)j * ldk + 8 * g) = vl;
@@ -760,8 +777,13 @@ dense_gemm_h_kernel(int M, int N, int ldk, const void* __restrict__ Ahv, const v
                     const void* __restrict__ Bhv, const void* __restrict__ Blv, float* C, int ldc, int tiles_m,
                     int tiles_n)
 {
-    using h_t = typename std::conditional<MODE == kUpdFP16, _Float16, __bf16>::type;
-    using v8_t = typename std::conditional<MODE == kUpdFP16, h8_t, bf8_t>::type;
+    // element type and the 8 elements one lane feeds an MFMA (16 bytes, or
+    // 8 for FP8)
+    using h_t = typename std::conditional<
+        MODE == kUpdFP16, _Float16,
+        typename std::conditional<MODE == kUpdFP8, unsigned char, __bf16>::type>::type;
+    using v8_t = typename std::conditional<
+        MODE == kUpdFP16, h8_t, typename std::conditional<MODE == kUpdFP8, long, bf8_t>::type>::type;
     constexpr int kSplit = (MODE == kUpdBF16x3) ? 2 : 1;
     constexpr int BK = 64;
     constexpr int LDK = BK + 8;
@@ -850,6 +872,8 @@ dense_gemm_h_kernel(int M, int N, int ldk, const void* __restrict__ Ahv, const v
                 for (int ti = 0; ti < 4; ++ti) {
                     if constexpr (MODE == kUpdFP16)
                         acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                    else if constexpr (MODE == kUpdFP8)
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
                     else
                         acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
                 }
@@ -995,7 +1019,9 @@ template <int MODE>
 magma_int_t update_h_launch(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc,
                             const DenseWork& w, hipStream_t st)
 {
-    using h_t = typename std::conditional<MODE == kUpdFP16, _Float16, __bf16>::type;
+    using h_t = typename std::conditional<
+        MODE == kUpdFP16, _Float16,
+        typename std::conditional<MODE == kUpdFP8, unsigned char, __bf16>::type>::type;
     if (M <= 0 || N <= 0) return MAGMA_SUCCESS;
     const int ldk = (int)roundup(K, 64);
     if (ldk > kDenseMaxNB || (size_t)M * ldk > w.h_elems || (size_t)N * ldk > w.h_elems) return MAGMA_ERR_NOT_SUPPORTED;
@@ -1029,6 +1055,7 @@ magma_int_t update_launch(int mode, int M, int N, int K, const T* A, int lda, co
             case kUpdBF16:   return update_h_launch<kUpdBF16>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
             case kUpdFP16:   return update_h_launch<kUpdFP16>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
             case kUpdBF16x3: return update_h_launch<kUpdBF16x3>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
+            case kUpdFP8:    return update_h_launch<kUpdFP8>(M, N, K, A, lda, B, ldb, C, ldc, w, st);
             default:         return MAGMA_ERR_NOT_SUPPORTED;
         }
     }
@@ -1225,7 +1252,7 @@ magma_int_t dense_getrf(int mode, int m, int n, double* dA, int ldda, int* d_ipi
 magma_int_t dense_sgetrf(int mode, int m, int n, float* dA, int ldda, int* d_ipiv, int* d_info, const DenseWork& w,
                          hipStream_t stream, int nb, hipStream_t side)
 {
-    if (mode != kUpdF32 && mode != kUpdBF16 && mode != kUpdFP16 && mode != kUpdBF16x3)
+    if (mode != kUpdF32 && mode != kUpdBF16 && mode != kUpdFP16 && mode != kUpdBF16x3 && mode != kUpdFP8)
         return MAGMA_ERR_NOT_SUPPORTED;
     return dense_getrf_t<float>(mode, nb > 0 ? nb : DenseShape<float>::NB, m, n, dA, ldda, d_ipiv, d_info, w, stream,
                                 side);

@@ -15,6 +15,7 @@ enum UpdateMode {
     kUpdBF16 = 2,    // FP32 storage, BF16 inputs, FP32 accumulation
     kUpdFP16 = 3,    // FP32 storage, FP16 inputs, FP32 accumulation
     kUpdBF16x3 = 4,  // FP32 storage, hi/lo BF16 split (three products)
+    kUpdFP8 = 5,     // FP32 storage, FP8 (e4m3) inputs, FP32 accumulation (dense LU only)
 };
 
 // Matrix sizes of a batch: fixed (arrays null) or per matrix (device arrays).

@@ -489,6 +489,7 @@ static int mode_of(magma_refinement_t f)
         case Magma_PREC_SHT: case Magma_PREC_XSH_163232TC: return kUpdFP16;
         case Magma_PREC_SBT: return kUpdBF16;
         case Magma_PREC_SB3T: return kUpdBF16x3;
+        case Magma_PREC_SF8T: return kUpdFP8;
         default: return -1;
     }
 }
@@ -734,7 +735,7 @@ magma_int_t magma_dxgesv_iteref_batched(magma_int_t n, magma_int_t nrhs, double*
     else if (ldda < std::max(1, n)) info = -4;
     else if (lddb < std::max(1, n)) info = -6;
     else if (lddx < std::max(1, n)) info = -8;
-    else if (mode < 0) info = -9;
+    else if (mode < 0 || mode == kUpdFP8) info = -9;     // the FP8 experiment runs on the dense LU only
     else if (solver_type != Magma_REFINE_IRSTRS && solver_type != Magma_REFINE_GMSTRS) info = -10;
     else if (itermax < 1) info = -11;
     else if (batchCount < 0) info = -14;
@@ -779,7 +780,7 @@ magma_int_t magma_dxgesv_gmres_gpu(magma_trans_t trans, magma_int_t n, magma_int
     }
     if (n == 0 || nrhs == 0) return 0;
     const bool s_plain = solver_type == Magma_REFINE_IRSTRS || solver_type == Magma_REFINE_GMSTRS;
-    if (!s_plain || n > kDenseIrMinN || !getrf_blocked_supported(n)) {
+    if (!s_plain || mode == kUpdFP8 || n > kDenseIrMinN || !getrf_blocked_supported(n)) {
         // one large system, or a mode the batched engine does not run: the
         // dense LU on the whole device
         const double t0 = magma_wtime();

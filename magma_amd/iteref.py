@@ -14,6 +14,7 @@ FACTO = {
     "fp16": _ffi.Magma_PREC_SHT,
     "bf16": _ffi.Magma_PREC_SBT,
     "bf16x3": _ffi.Magma_PREC_SB3T,
+    "fp8": _ffi.Magma_PREC_SF8T,        # experiment: one-system drivers only
 }
 # short names of the refinement solvers (include/magma_types.h Magma_REFINE_*):
 # "ir" IRSTRS, "gmres" GMSTRS, and the reference's own names in lower case

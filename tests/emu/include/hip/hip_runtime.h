@@ -256,6 +256,58 @@ inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_f16(emu_h8 a, emu_h8 b, emu_f4 
 {
     return emu_mfma_x32(a, b, c);
 }
+// FP8 (OCP e4m3fn on gfx950): round to nearest even, no infinity; beyond the
+// largest finite value the result is NaN (the conversion runs without clamp).
+inline float emu_fp8_dec(unsigned b)
+{
+    b &= 0xffu;
+    const float sg = (b & 0x80u) ? -1.f : 1.f;
+    const unsigned e = (b >> 3) & 0xfu, m = b & 7u;
+    if (e == 15u && m == 7u) return NAN;
+    if (e == 0u) return sg * std::ldexp((float)m, -9);
+    return sg * std::ldexp(1.f + (float)m / 8.f, (int)e - 7);
+}
+inline unsigned emu_fp8_enc(float x)
+{
+    if (x != x) return 0x7fu;
+    const unsigned sg = std::signbit(x) ? 0x80u : 0u;
+    const float a = std::fabs(x);
+    if (a >= 464.f) return sg | 0x7fu;
+    if (a < 0.015625f) {
+        const float q = std::nearbyint(a * 512.f);          // units of 2^-9, ties to even
+        return sg | (unsigned)q;                            // q == 8 is the smallest normal
+    }
+    int e;
+    const float fr = std::frexp(a, &e);                     // a = fr * 2^e, fr in [0.5, 1)
+    int ex = e - 1;                                         // a = (2 fr) * 2^ex
+    float q = std::nearbyint((2.f * fr - 1.f) * 8.f);
+    if (q >= 8.f) { q = 0.f; ++ex; }
+    if (ex + 7 > 15 || (ex + 7 == 15 && q > 6.f)) return sg | 0x7eu;   // 448
+    return sg | ((unsigned)(ex + 7) << 3) | (unsigned)q;
+}
+inline int __builtin_amdgcn_cvt_pk_fp8_f32(float a, float b, int old, bool hi)
+{
+    const unsigned v = emu_fp8_enc(a) | (emu_fp8_enc(b) << 8);
+    const unsigned o = (unsigned)old;
+    return (int)(hi ? ((o & 0x0000ffffu) | (v << 16)) : ((o & 0xffff0000u) | v));
+}
+inline float __builtin_amdgcn_cvt_f32_fp8(int src, int byte_sel)
+{
+    return emu_fp8_dec((unsigned)src >> (8 * byte_sel));
+}
+struct emu_f8x8 {
+    float v[8];
+    float operator[](int i) const { return v[i]; }
+};
+inline emu_f4 __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(long a, long b, emu_f4 c, int, int, int)
+{
+    emu_f8x8 fa, fb;
+    for (int v = 0; v < 8; ++v) {
+        fa.v[v] = emu_fp8_dec((unsigned)((unsigned long)a >> (8 * v)));
+        fb.v[v] = emu_fp8_dec((unsigned)((unsigned long)b >> (8 * v)));
+    }
+    return emu_mfma_x32(fa, fb, c);
+}
 
 // Runtime API (host memory stands in for device memory).
 hipError_t hipMalloc(void** p, size_t bytes);

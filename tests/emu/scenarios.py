@@ -30,7 +30,7 @@ libc.malloc.argtypes = [ctypes.c_size_t]
 libc.free.argtypes = [ctypes.c_void_p]
 vp, i32 = ctypes.c_void_p, ctypes.c_int
 
-PREC = {"fp32": 900, "fp16": 905, "bf16": 940, "bf16x3": 941}
+PREC = {"fp32": 900, "fp16": 905, "bf16": 940, "bf16x3": 941, "fp8": 942}
 SOLV = {"ir": 920, "gmres": 924, "irstrs": 920, "irdtrs": 921, "irgmstrs": 922, "irgmdtrs": 923, "gmstrs": 924,
         "gmdtrs": 925, "gmgmstrs": 926, "gmgmdtrs": 927}
 
@@ -427,7 +427,7 @@ def scenario_dense_sgetrf_prec(m, n, facto):
     # the residual grows like the update format's unit roundoff times n
     # (measured ratio res / (u n) is 0.02-0.15 for all three formats); an
     # indexing mistake in the update gives O(1) or worse
-    u = {"fp32": 2.0**-24, "bf16x3": 2.0**-16, "fp16": 2.0**-11, "bf16": 2.0**-8}[facto]
+    u = {"fp32": 2.0**-24, "bf16x3": 2.0**-16, "fp16": 2.0**-11, "bf16": 2.0**-8, "fp8": 2.0**-4}[facto]
     tol = max(2e-4, 0.5 * u * max(m, n))
     print(f"dense sgetrf {facto} {m}x{n} rc {rc} info {info.value} res {res:.2e}", flush=True)
     check(rc == 0 and info.value == 0 and res < tol, f"dense sgetrf {facto} {m}x{n}")

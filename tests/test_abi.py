@@ -76,3 +76,45 @@ def test_wtime_monotone(lib):
     a = lib.magma_wtime()
     b = lib.magma_wtime()
     assert b >= a
+
+
+def test_refinement_argument_codes(lib):
+    """magma_dxgesv_gmres_gpu (src/dxgesv_gmres_gpu.cpp:219-359): every
+    Magma_REFINE_* solver type is accepted, anything else is -15; an unknown
+    factorisation type is -14; n = 0 returns before any device work."""
+    it, info = ctypes.c_int(7), ctypes.c_int(7)
+    ft = ctypes.c_double(0)
+    ip = (ctypes.c_int * 4)()
+
+    def call(trans, n, facto, solver, ldda=4):
+        return lib.magma_dxgesv_gmres_gpu(trans, n, 1, None, ldda, ip, None, None, max(n, 1), None, max(n, 1),
+                                          None, None, facto, solver, ctypes.byref(it), ctypes.byref(info),
+                                          ctypes.byref(ft))
+    assert call(999, 4, ffi.Magma_PREC_SS, ffi.Magma_REFINE_IRSTRS) == -1
+    assert call(111, -1, ffi.Magma_PREC_SS, ffi.Magma_REFINE_IRSTRS) == -2
+    assert call(111, 4, ffi.Magma_PREC_SS, ffi.Magma_REFINE_IRSTRS, ldda=3) == -5
+    assert call(111, 4, 12345, ffi.Magma_REFINE_IRSTRS) == -14
+    assert call(111, 4, ffi.Magma_PREC_SS, 919) == -15
+    assert call(111, 4, ffi.Magma_PREC_SS, 928) == -15
+    for solver in range(ffi.Magma_REFINE_IRSTRS, ffi.Magma_REFINE_GMGMDTRS + 1):
+        for facto in (ffi.Magma_PREC_SS, ffi.Magma_PREC_SHT, ffi.Magma_PREC_SBT, ffi.Magma_PREC_SB3T,
+                      ffi.Magma_PREC_SF8T):
+            assert call(111, 0, facto, solver) == 0
+            assert (it.value, info.value) == (0, 0)
+    # the batched solver runs IRSTRS / GMSTRS with 16-bit or FP32 updates;
+    # the FP8 experiment is for the one-system drivers only
+    q = ctypes.c_void_p(0)
+    assert lib.magma_dxgesv_iteref_batched(4, 1, None, 4, None, 4, None, 4, ffi.Magma_PREC_SF8T,
+                                           ffi.Magma_REFINE_IRSTRS, 10, None, None, 1, q) == -9
+    assert lib.magma_dxgesv_iteref_batched(4, 1, None, 4, None, 4, None, 4, ffi.Magma_PREC_SBT,
+                                           ffi.Magma_REFINE_IRDTRS, 10, None, None, 1, q) == -10
+
+
+def test_prec_lu_and_adjoint_argument_codes(lib):
+    info = ctypes.c_int(0)
+    ip = (ctypes.c_int * 4)()
+    assert lib.magma_sgetrf_prec_gpu(4, 4, None, 4, ip, ctypes.byref(info), 12345) == -7
+    assert lib.magma_sgetrf_prec_gpu(-1, 4, None, 4, ip, ctypes.byref(info), ffi.Magma_PREC_SF8T) == -1
+    assert lib.magma_adjoint_outer_batched(2, 4, 1, None, 4, 4, None, 4, 4, None, 4, 16, 1, None) == -1
+    assert lib.magma_adjoint_outer_batched(8, 4, 1, None, 3, 4, None, 4, 4, None, 4, 16, 1, None) == -5
+    assert lib.magma_adjoint_outer_batched(8, 0, 1, None, 1, 4, None, 1, 4, None, 1, 16, 1, None) == 0

@@ -44,7 +44,8 @@ def test_kernels_under_asan(emu_lib):
 # on one another, so the emulation runs several workgroups interleaved.
 DENSE_SCENARIOS = ["dgetrf:300:280", "dgetrf:600:700", "sgetrf:520:530", "sgetrfp:200:200:bf16x3",
                    "sgetrfp:300:260:fp16", "sgetrfp:260:300:bf16", "dgetrs:300:3:0", "dgetrs:300:3:1",
-                   "dgetrs:400:70:1", "dhgesv:600:fp32:irdtrs", "dhgesv:300:fp32:gmgmstrs"]
+                   "dgetrs:400:70:1", "dhgesv:600:fp32:irdtrs", "dhgesv:300:fp32:gmgmstrs",
+                   "sgetrfp:200:200:fp8"]
 
 
 @pytest.mark.timeout(900)

@@ -158,6 +158,49 @@ def test_dense_sgetrf(dn):
     assert res < 30 * np.finfo(np.float32).eps, res
 
 
+@pytest.mark.parametrize("facto,u", [("bf16", 2.0 ** -8), ("fp16", 2.0 ** -11), ("bf16x3", 2.0 ** -16),
+                                     ("fp8", 2.0 ** -4)])
+def test_dense_sgetrf_prec_residual(dn, facto, u):
+    """magma_sgetrf_prec_gpu: the FP32 LU with 16-bit (or, as an experiment,
+    FP8 e4m3) trailing updates has finite factors and a residual at the level
+    of the update format: ||PA - LU||_F / ||A||_F below 40 u, u its unit
+    roundoff (measured at this order: 19 u for BF16 and FP16, 9 u for
+    split-BF16, 21 u for FP8), with the pivots a valid permutation."""
+    rng = np.random.default_rng(11)
+    n = 1024
+    Arow = rng.uniform(-1, 1, (n, n))
+    dA = torch.from_numpy(cm(Arow).astype(np.float32)).cuda()
+    ipiv, info = dn.sgetrf_prec_gpu(dA, facto)
+    assert info == 0
+    LU = dA.cpu().numpy().astype(np.float64).T
+    assert np.isfinite(LU).all()
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    PA = Arow.copy()
+    for j, p in enumerate(ipiv):
+        assert j + 1 <= p <= n
+        PA[[j, p - 1]] = PA[[p - 1, j]]
+    res = np.linalg.norm(PA - L @ U) / np.linalg.norm(PA)
+    print(f"{facto}: residual {res:.3e} (u = {u:.3e})")
+    assert res < 40 * u, res
+
+
+def test_dense_fp8_refinement(dn, oracle):
+    """The FP8 experiment inside magma_dxgesv_gmres_gpu: whatever the FP8
+    factors are worth as a preconditioner, the answer is of FP64 quality
+    (converged to MAGMA's test, or the FP64 fallback with iter < 0)."""
+    from magma_amd import iteref
+    n = 1024
+    A = svd_geo_gpu(n, 10.0, seed=4)
+    b = torch.from_numpy(np.random.default_rng(4).uniform(-1, 1, (1, n))).cuda()
+    for solver in ("ir", "gmres"):
+        X, ipiv, dipiv, it, info = iteref.dhgesv_iteref_gpu(A.clone(), b.clone(), facto="fp8", solver=solver)
+        assert info == 0
+        be = normwise_be(A, X, b)
+        assert (it >= 0 and be <= 1e-14) or (it < 0 and be <= TESTER_TOL * n), (solver, it, be)
+        print(f"fp8-{solver}: iter {it} be {be:.2e}")
+
+
 def test_dense_argument_codes(dn):
     from magma_amd import _ffi
     lib = _ffi.load()
