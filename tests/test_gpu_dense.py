@@ -94,6 +94,23 @@ def test_dense_lookahead_matches_serial(dn, monkeypatch, prec, m, n):
     assert np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("prec,lookahead", [("d", False), ("d", True), ("s", True)])
+def test_dense_exact_permuted_upper(dn, monkeypatch, prec, lookahead):
+    """A = P U with U integer upper triangular: every multiplier is 0 and all
+    arithmetic exact, so the dense LU must return U bit for bit and ipiv must
+    sort A's rows back into U (the row interchanges checked exactly), with
+    and without the two-stream schedule."""
+    from test_gpu_lu import apply_ipiv, permuted_upper
+    monkeypatch.setenv("MAGMA_DENSE_LOOKAHEAD_MIN", "0" if lookahead else "1000000000")
+    n = 1300
+    Arow, U = permuted_upper(np.random.default_rng(3), n)
+    dt = np.float64 if prec == "d" else np.float32
+    LU, ipiv, info = run_getrf(dn, Arow, dtype=dt)
+    assert info == 0
+    assert np.array_equal(LU.T.astype(np.float64), U)
+    assert np.array_equal(apply_ipiv(Arow, ipiv), U)
+
+
 def test_dense_native_matches_gpu(dn):
     rng = np.random.default_rng(3)
     Arow = rng.uniform(-1, 1, (777, 777))

@@ -90,6 +90,40 @@ def test_getrf_rectangular(mb, oracle, m, n):
     check_getrf(oracle, A, LU, ipiv, info, m, n, f"{m}x{n}")
 
 
+def permuted_upper(rng, n):
+    """A = P U: U upper triangular with small-integer entries (diagonal
+    +-1..9, above it -4..4), row i of U placed at row perm[i]. Partial
+    pivoting finds exactly one nonzero candidate per column, every multiplier
+    is 0, and all arithmetic is exact: the factorisation must return U
+    bit for bit, with ipiv sorting the rows back (SURVEY's exact check of the
+    row interchanges)."""
+    U = np.triu(rng.integers(-4, 5, (n, n))).astype(np.float64)
+    U[np.arange(n), np.arange(n)] = rng.integers(1, 10, n) * rng.choice([-1.0, 1.0], n)
+    perm = rng.permutation(n)
+    A = np.empty_like(U)
+    A[perm] = U
+    return A, U
+
+
+def apply_ipiv(A, ipiv):
+    PA = A.copy()
+    for j, p in enumerate(ipiv):
+        PA[[j, p - 1]] = PA[[p - 1, j]]
+    return PA
+
+
+@pytest.mark.parametrize("n", [16, 64, 128, 200, 300, 512, 777])
+def test_getrf_exact_permuted_upper(mb, n):
+    rng = np.random.default_rng(n)
+    rows = [permuted_upper(rng, n) for _ in range(3)]
+    A = np.stack([a.T.copy() for a, _ in rows])          # column storage
+    LU, ipiv, info = run_getrf(mb, A)
+    assert (info == 0).all()
+    for s, (Arow, U) in enumerate(rows):
+        assert np.array_equal(LU[s].T, U), f"system {s}: factors differ from U"
+        assert np.array_equal(apply_ipiv(Arow, ipiv[s]), U), f"system {s}: ipiv does not sort the rows back"
+
+
 def test_getrf_golden_fixtures(mb, oracle, golden):
     for name in golden_cases(golden):
         Ar = golden[f"{name}/A"]
