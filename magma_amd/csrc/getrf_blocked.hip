@@ -568,6 +568,131 @@ static magma_int_t panel_launch(int R, const SizeArgs& sz, int k, T** dA_array, 
     return MAGMA_SUCCESS;
 }
 
+// The full trailing update of larger systems (FP64 or FP32): 128 x 128
+// output tiles, so each A21 / U12 strip is read half as often as with the
+// 64 x 64 tiles above. Four waves of 64 x 64 (4 x 4 MFMA tiles each); K
+// staged through LDS 16 at a time, double-buffered, one barrier per stage;
+// the product is formed transposed (the same store pattern as above).
+// Grid: x = tile of the system (column-major over the tile grid), y = system.
+template <typename T>
+__global__ void __launch_bounds__(256, 2)
+gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
+{
+    constexpr int TM = 128, BK = 16, LDP = TM + 4;
+    const int b = sz.mat(blockIdx.y);
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    if (k >= minmn) return;
+    const int jb = min(nb, minmn - k);
+    const int M = m - k - jb, N = n - k - jb, K = jb;
+    if (M <= 0 || N <= 0) return;
+    const int tiles_m = (M + TM - 1) / TM;
+    const int tid = blockIdx.x;
+    if (tid >= tiles_m * ((N + TM - 1) / TM)) return;
+    const int m0 = (tid % tiles_m) * TM, n0 = (tid / tiles_m) * TM;
+    T* Ab = dA_array[b];
+    const T* A = Ab + (k + jb) + (size_t)k * ld;          // A21
+    const T* B = Ab + k + (size_t)(k + jb) * ld;          // U12
+    T* C = Ab + (k + jb) + (size_t)(k + jb) * ld;
+    const int t = threadIdx.x;
+    const int w = t / kWave, l = t % kWave;
+    const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+    const int lr = l % 16, lq = l / 16;
+    __shared__ T sA[2][BK][LDP];
+    __shared__ T sB[2][BK][LDP];
+    using acc_t = typename std::conditional<std::is_same<T, double>::value, double4_t, float4_t>::type;
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+    T ra[8], rb[8];
+    const int ai = t & 127, ak = t >> 7;       // A tile: row ai, k = ak + 2q
+    const int bk = t & 15, bj = t >> 4;        // B tile: k = bk, column bj + 16q
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int kk = ak + 2 * q;
+            ra[q] = (m0 + ai < M && k0 + kk < K) ? A[(size_t)(m0 + ai) + (size_t)(k0 + kk) * ld] : T(0);
+            const int jj = bj + 16 * q;
+            rb[q] = (k0 + bk < K && n0 + jj < N) ? B[(size_t)(k0 + bk) + (size_t)(n0 + jj) * ld] : T(0);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            sA[buf][ak + 2 * q][ai] = ra[q];
+            sB[buf][bk][bj + 16 * q] = rb[q];
+        }
+    };
+    const int nst = (K + BK - 1) / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) gload((st + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T pf[4], qf[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                pf[x] = sB[buf][kk + lq][wc + 16 * x + lr];
+                qf[x] = sA[buf][kk + lq][wr + 16 * x + lr];
+            }
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti) {
+                    if constexpr (std::is_same<T, double>::value)
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                    else
+                        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x4f32(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
+                }
+        }
+        if (st + 1 < nst) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) {
+            const int row = m0 + wr + 16 * ti + lr;
+            if (row >= M) continue;
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) {
+                const int cc = std::is_same<T, double>::value ? (4 * rg + lq) : (4 * lq + rg);
+                const int col = n0 + wc + 16 * tj + cc;
+                if (col < N) {
+                    T* cp = C + (size_t)row + (size_t)col * ld;
+                    *cp = *cp - acc[tj][ti][rg];
+                }
+            }
+        }
+}
+
+// The full trailing update after a block of NB columns: 128-wide tiles for
+// the FP64 / FP32 updates of systems with at least 128 trailing rows, the
+// 64-wide kernel otherwise (and for the 16-bit formats).
+template <typename T, int NB, int MODE>
+static magma_int_t trailing_update(const SizeArgs& sz, int k, int rest_m, int rest_n, T** dA_array, int batch,
+                                   hipStream_t stream)
+{
+    if constexpr (MODE == kUpdF64 || MODE == kUpdF32) {
+        if (rest_m >= 128 && rest_n >= 128) {
+            const int tiles = ceildiv(rest_m, 128) * ceildiv(rest_n, 128);
+            gemm_update128_kernel<T><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
+            MGI_LAUNCH_CHECK();
+            return MAGMA_SUCCESS;
+        }
+    }
+    dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
+    gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array, 0);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 template <typename T, int NB, int MODE>
 static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n, int max_minmn,
                                  T** dA_array, int** ipiv_array, int* info_array, int batch,
@@ -588,9 +713,8 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
             trsm_kernel<T, NB><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array, 0);
             MGI_LAUNCH_CHECK();
             if (rest_m > 0) {
-                dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
-                gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array, 0);
-                MGI_LAUNCH_CHECK();
+                const magma_int_t urc = trailing_update<T, NB, MODE>(sz, k, rest_m, rest_n, dA_array, batch, stream);
+                if (urc != MAGMA_SUCCESS) return urc;
             }
         }
     }
@@ -646,9 +770,9 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
             trsm_kernel<T, 2 * H><<<dim3(ceildiv(rest_n, kTrsmCols), batch), 256, 0, stream>>>(sz, k, dA_array, 0);
             MGI_LAUNCH_CHECK();
             if (rest_m > 0) {
-                dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
-                gemm_update_kernel<T, 2 * H, MODE><<<grid, 256, 0, stream>>>(sz, k, 2 * H, dA_array, 0);
-                MGI_LAUNCH_CHECK();
+                const magma_int_t urc =
+                    trailing_update<T, 2 * H, MODE>(sz, k, rest_m, rest_n, dA_array, batch, stream);
+                if (urc != MAGMA_SUCCESS) return urc;
             }
         }
     }
