@@ -393,7 +393,8 @@ magma_int_t magma_dbgesv_iteref_gpu(magma_trans_t trans, magma_int_t n, magma_in
  * FP64 fallback are each one batched launch sequence on the queue's stream,
  * and the call returns after the refinement has finished). A_i and B_i are
  * read only; X_i is written. facto_type selects the low-precision LU (SHT, SBT, SB3T, SS),
- * solver_type IRSTRS or GMSTRS; itermax bounds refinement steps.
+ * solver_type IRSTRS or GMSTRS (FGMRES restarted every 50 iterations, which
+ * bounds the Krylov basis of the whole batch); itermax bounds refinement steps.
  * iter_array[i] follows the iter convention of magma_dhgesv_iteref_gpu;
  * a system that does not converge is re-solved in FP64 inside the launch.
  * info_array[i] > 0 if the FP64 fallback found an exactly-zero pivot. */
@@ -422,8 +423,8 @@ magma_int_t magma_adjoint_outer_batched(magma_int_t elem, magma_int_t n, magma_i
  * hipMalloc): the LU of magma_dgetrf_native (same kernels, same results),
  * with its scratch in a fixed 1 MB device array of the library instead of
  * allocations. d_ipiv (min(m,n) entries) and d_info are device memory; the
- * call is asynchronous on queue's stream. m <= 256 * (compute units, at most
- * 256), i.e. 65,536 rows on MI355X: larger m returns MAGMA_ERR_NOT_SUPPORTED.
+ * call is asynchronous on queue's stream. m <= 65,536 rows (and at most 256
+ * rows per compute unit): larger m returns MAGMA_ERR_NOT_SUPPORTED.
  * Calls that overlap on one device share that array and race: serialise them
  * (one queue, or events between queues). A wait that exceeds its time limit
  * sets *d_info = MAGMA_ERR_UNKNOWN. */

@@ -13,8 +13,9 @@ namespace mgi {
 
 namespace {
 
-// Panel workgroups the scratch holds: m <= 256 * 256 = 65,536 rows.
-constexpr int kNaGmax = 256;
+// Panel workgroups the scratch holds: m <= 65,536 rows.
+constexpr int kNaMaxRows = 65536;
+constexpr int kNaGmax = kNaMaxRows / kDensePanelRows;
 constexpr size_t kNaBytes = 1024 * 1024;
 
 // 1 MB: the panel's slot words (528 KB), one block inverse (128 KB), pairs.
@@ -73,8 +74,8 @@ magma_int_t magma_dgetrf_native_noalloc(magma_int_t m, magma_int_t n, magmaDoubl
     const int g = dense_panel_gmax(m);
     if (g > kNaGmax || g > device_cu_count()) {
         std::fprintf(stderr, "magma_dgetrf_native_noalloc: m = %d rows exceed its scratch limit of %d rows "
-                     "(256 per compute unit, at most %d)\n", (int)m, 256 * std::min(kNaGmax, device_cu_count()),
-                     256 * kNaGmax);
+                     "(%d per compute unit, at most %d)\n", (int)m,
+                     kDensePanelRows * std::min(kNaGmax, device_cu_count()), kDensePanelRows, kNaMaxRows);
         return MAGMA_ERR_NOT_SUPPORTED;
     }
     if (dense_work_bytes(m, n, (int)sizeof(double), kNaGmax) > kNaBytes) return MAGMA_ERR_NOT_SUPPORTED;

@@ -673,14 +673,16 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
 }
 
 // The full trailing update after a block of NB columns: 128-wide tiles for
-// the FP64 / FP32 updates of systems with at least 128 trailing rows, the
-// 64-wide kernel otherwise (and for the 16-bit formats).
+// the FP64 / FP32 updates while the trailing matrix is at least 256 x 256
+// (below that the 64-wide tiles give each system enough workgroups: at
+// n = 256 the 128-wide tiles were 14 % slower), the 64-wide kernel otherwise
+// and for the 16-bit formats.
 template <typename T, int NB, int MODE>
 static magma_int_t trailing_update(const SizeArgs& sz, int k, int rest_m, int rest_n, T** dA_array, int batch,
                                    hipStream_t stream)
 {
     if constexpr (MODE == kUpdF64 || MODE == kUpdF32) {
-        if (rest_m >= 128 && rest_n >= 128) {
+        if (rest_m >= 256 && rest_n >= 256) {
             const int tiles = ceildiv(rest_m, 128) * ceildiv(rest_n, 128);
             gemm_update128_kernel<T><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
             MGI_LAUNCH_CHECK();

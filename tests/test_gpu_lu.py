@@ -95,8 +95,8 @@ def permuted_upper(rng, n):
     +-1..9, above it -4..4), row i of U placed at row perm[i]. Partial
     pivoting finds exactly one nonzero candidate per column, every multiplier
     is 0, and all arithmetic is exact: the factorisation must return U
-    bit for bit, with ipiv sorting the rows back (SURVEY's exact check of the
-    row interchanges)."""
+    bit for bit, with ipiv sorting the rows back (an exact check of the row
+    interchanges on integer data)."""
     U = np.triu(rng.integers(-4, 5, (n, n))).astype(np.float64)
     U[np.arange(n), np.arange(n)] = rng.integers(1, 10, n) * rng.choice([-1.0, 1.0], n)
     perm = rng.permutation(n)
