@@ -125,18 +125,35 @@ magma_int_t magma_dgesv_batched(magma_int_t n, magma_int_t nrhs, double** dA_arr
 
 void magma_get_dgetrf_batched_nbparam(magma_int_t n, magma_int_t* nb, magma_int_t* recnb)
 {
-    // Panel width of the tiled kernels: one 16-column MFMA tile.
-    (void)n;
-    if (nb) *nb = 16;
-    if (recnb) *recnb = 16;
+    // The blocking the square n x n batched LU uses: the register-tile kernel
+    // factors 16-column panels; the LDS kernel factors the whole matrix one
+    // column at a time; the blocked LU factors nb-wide blocks of 32-wide
+    // panels.
+    int b = 16, r = 16;
+    if (!mgi::getrf_tile_supported(n)) {
+        if (mgi::getrf_lds_bytes(n, n) <= mgi::max_dynamic_lds()) {
+            b = std::max(1, n);
+            r = 1;
+        } else {
+            b = std::max(mgi::getrf_blocked_nb(n), 1);
+            r = std::min(b, 32);
+        }
+    }
+    if (nb) *nb = b;
+    if (recnb) *recnb = r;
 }
 
 void magma_get_dgetrf_vbatched_nbparam(magma_int_t max_m, magma_int_t max_n, magma_int_t* nb, magma_int_t* recnb)
 {
-    (void)max_m;
-    (void)max_n;
-    if (nb) *nb = 16;
-    if (recnb) *recnb = 16;
+    // The vbatched LU puts systems up to 128 on the register tile (16-column
+    // panels) and larger ones on the blocked LU sized by the largest row count.
+    int b = 16, r = 16;
+    if (max_m > 128 || max_n > 128) {
+        b = std::max(mgi::getrf_blocked_nb(max_m), 1);
+        r = std::min(b, 32);
+    }
+    if (nb) *nb = b;
+    if (recnb) *recnb = r;
 }
 
 }  // extern "C"

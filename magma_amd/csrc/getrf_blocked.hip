@@ -71,11 +71,11 @@ panel_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int** __restrict__ i
     const int mloc = m - k;
     const int t = threadIdx.x;
     const int w = t / kWave;
-    T* A = dA_array[b] + k + (size_t)k * ld;     // panel origin
+    auto A = gptr(dA_array[b]) + k + (size_t)k * ld;     // panel origin
     // a null pivot array asks for the factorisation without pivoting
     // (src/zgetrf_nopiv_batched.cpp): the diagonal entry is the pivot
     const bool nopiv = ipiv_array == nullptr;
-    int* ipiv = nopiv ? nullptr : ipiv_array[b];
+    auto ipiv = gptr(nopiv ? nullptr : ipiv_array[b]);
 
     T a[R][NB];
 #pragma unroll
@@ -234,7 +234,7 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
     if (cbeg >= span) return;
     const int t = threadIdx.x;
     if (t < jb) {
-        const int p = ipiv_array[b][k + t] - 1;
+        const int p = gptr(ipiv_array[b])[k + t] - 1;
         s_piv[t] = (p > k + t && p < m) ? p : k + t;
     }
     __syncthreads();
@@ -255,16 +255,36 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
     }
     const bool mv0 = dst0 >= 0 && src0 != dst0, mv1 = dst1 >= 0 && src1 != dst1;
     const int cend = min(span, cbeg + kLaswpCols);
-    for (int c = cbeg + w; c < cend; c += 256 / kWave) {
-        const int col = mode == 0 ? (c < k ? c : c + jb) : lo + c;
-        T* A = dA_array[b] + (size_t)col * ld;
-        T v0 = T(0), v1 = T(0);
-        if (mv0) v0 = A[src0];
-        if (mv1) v1 = A[src1];
-        __builtin_amdgcn_wave_barrier();            // all sources read before any store
-        if (mv0) A[dst0] = v0;
-        if (mv1) A[dst1] = v1;
-        __builtin_amdgcn_wave_barrier();
+    // each wave moves its kLaswpCols / 4 columns at once: every source is
+    // read before any store (the columns are disjoint, so the loads of all
+    // of them are in flight together)
+    constexpr int kPer = kLaswpCols / (256 / kWave);
+    T v0[kPer], v1[kPer];
+    auto base = gptr(dA_array[b]);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int c = cbeg + w + u * (256 / kWave);
+        v0[u] = T(0);
+        v1[u] = T(0);
+        if (c < cend) {
+            const int col = mode == 0 ? (c < k ? c : c + jb) : lo + c;
+            auto A = base + (size_t)col * ld;
+            if (mv0) v0[u] = A[src0];
+            if (mv1) v1[u] = A[src1];
+        }
+    }
+    // every load of the wave has returned before the first store issues
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0); expcnt, lgkmcnt unconstrained
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int c = cbeg + w + u * (256 / kWave);
+        if (c < cend) {
+            const int col = mode == 0 ? (c < k ? c : c + jb) : lo + c;
+            auto A = base + (size_t)col * ld;
+            if (mv0) A[dst0] = v0[u];
+            if (mv1) A[dst1] = v1[u];
+        }
     }
 }
 
@@ -299,7 +319,7 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
     const int c0 = blockIdx.x * kTrsmCols;
     if (c0 >= ncols) return;
     const int t = threadIdx.x;
-    const T* L = dA_array[b] + k + (size_t)k * ld;
+    auto L = gptr(dA_array[b]) + k + (size_t)k * ld;
     for (int idx = t; idx < NB * NB; idx += 256) {
         const int i = idx % NB, j = idx / NB;
         sL[idx] = (i < jb && j < jb) ? L[i + (size_t)j * ld] : T(0);
@@ -311,7 +331,7 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
     for (int cb = c0 + w * CPW; cb < cend; cb += (256 / kWave) * CPW) {
         const int c = cb + cq;
         const bool on = c < cend && i < jb;
-        T* X = dA_array[b] + k + (size_t)(k + jb + c) * ld;
+        auto X = gptr(dA_array[b]) + k + (size_t)(k + jb + c) * ld;
         T x = on ? X[i] : T(0);
 #pragma unroll
         for (int j = 0; j < NB - 1; ++j) {
@@ -372,10 +392,10 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
     }
     const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
     if (r0 >= mr || c0 >= nc) return;
-    T* A = dA_array[b];
-    const T* A21 = A + (k + jb) + (size_t)k * ld;
-    const T* A12 = A + k + (size_t)(k + jb) * ld;
-    T* C = A + (k + jb) + (size_t)(k + jb) * ld;
+    auto A = gptr(dA_array[b]);
+    auto A21 = A + (k + jb) + (size_t)k * ld;
+    auto A12 = A + k + (size_t)(k + jb) * ld;
+    auto C = A + (k + jb) + (size_t)(k + jb) * ld;
     const int t = threadIdx.x;
     for (int idx = t; idx < KP * 64; idx += 256) {
         const int r = idx % 64, kk = idx / 64;           // A21: walk down columns
@@ -419,7 +439,7 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
                 for (int r = 0; r < 4; ++r) {
                     const int col = c0 + wc + 16 * tj + 4 * r + lq;
                     if (col < nc) {
-                        T* cp = C + row + (size_t)col * ld;
+                        auto cp = C + row + (size_t)col * ld;
                         *cp = *cp - acc[ti][tj][r];
                     }
                 }
@@ -508,7 +528,7 @@ gemm_update_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mo
                 for (int r = 0; r < 4; ++r) {
                     const int col = c0 + wc + 16 * tj + 4 * lq + r;
                     if (col < nc) {
-                        T* cp = C + row + (size_t)col * ld;
+                        auto cp = C + row + (size_t)col * ld;
                         *cp = *cp - acc[ti][tj][r];
                     }
                 }
@@ -591,10 +611,10 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
     const int tid = blockIdx.x;
     if (tid >= tiles_m * ((N + TM - 1) / TM)) return;
     const int m0 = (tid % tiles_m) * TM, n0 = (tid / tiles_m) * TM;
-    T* Ab = dA_array[b];
-    const T* A = Ab + (k + jb) + (size_t)k * ld;          // A21
-    const T* B = Ab + k + (size_t)(k + jb) * ld;          // U12
-    T* C = Ab + (k + jb) + (size_t)(k + jb) * ld;
+    auto Ab = gptr(dA_array[b]);
+    auto A = Ab + (k + jb) + (size_t)k * ld;              // A21
+    auto B = Ab + k + (size_t)(k + jb) * ld;              // U12
+    auto C = Ab + (k + jb) + (size_t)(k + jb) * ld;
     const int t = threadIdx.x;
     const int w = t / kWave, l = t % kWave;
     const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
@@ -665,7 +685,7 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
                 const int cc = std::is_same<T, double>::value ? (4 * rg + lq) : (4 * lq + rg);
                 const int col = n0 + wc + 16 * tj + cc;
                 if (col < N) {
-                    T* cp = C + (size_t)row + (size_t)col * ld;
+                    auto cp = C + (size_t)row + (size_t)col * ld;
                     *cp = *cp - acc[tj][ti][rg];
                 }
             }
@@ -782,6 +802,7 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
 }
 
 bool getrf_blocked_supported(int max_m) { BlockedPlan p; return plan_for(max_m, p); }
+int getrf_blocked_nb(int max_m) { BlockedPlan p; return plan_for(max_m, p) ? p.nb : 0; }
 
 // Panel widths a given row count supports: the panel keeps R x NB values per
 // thread in registers, R = ceil(max_m / 256), and R * NB <= 64. Width 64

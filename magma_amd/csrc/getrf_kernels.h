@@ -78,6 +78,7 @@ magma_int_t getrs_generic_launch(int trans, int n, int nrhs, double** dA_array, 
 // getrf_blocked.hip: blocked panel/laswp/trsm/gemm steps, fixed size (m_arr
 // == nullptr) or per-matrix sizes in device arrays; max_m <= 2048.
 bool getrf_blocked_supported(int max_m);
+int getrf_blocked_nb(int max_m);   // panel width the blocked LU uses (0: unsupported)
 magma_int_t getrf_blocked_launch(int m, int n, int ldda, const int* m_arr, const int* n_arr, const int* ld_arr,
                                  int max_m, int max_n, double** dA_array, int** ipiv_array, int* info_array,
                                  int batch, hipStream_t stream);

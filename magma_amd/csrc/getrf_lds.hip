@@ -32,8 +32,8 @@ getrf_lds_kernel(int m, int n, T** __restrict__ dA_array, int ldda, int** __rest
     __shared__ T s_pv;
     const int tid = threadIdx.x;
     const int ld = m + 1;                  // odd stride: row walks spread over banks
-    T* A = dA_array[blockIdx.x];
-    int* ipiv = ipiv_array[blockIdx.x];
+    auto A = gptr(dA_array[blockIdx.x]);
+    auto ipiv = gptr(ipiv_array[blockIdx.x]);
 
     for (int idx = tid; idx < m * n; idx += kLdsThreads) {
         int i = idx % m, j = idx / m;

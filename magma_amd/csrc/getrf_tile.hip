@@ -52,11 +52,10 @@ constexpr int kTileThreads = 256;
 constexpr int kTileWaves = 4;
 constexpr int kTileOwners = 3;      // waves 0..2 hold tiles, wave 3 factors panels
 
-// Matrices are reached through pointer arrays, so the compiler sees generic
-// pointers; naming the global address space lets it use global loads with a
-// scalar base and a 32-bit lane offset instead of 64-bit flat addresses.
-typedef __attribute__((address_space(1))) double gdouble;
-__device__ __forceinline__ gdouble* as_global(double* p) { return (gdouble*)p; }
+// The matrices are held through global-address-space pointers (gptr), so the
+// compiler uses global loads with a scalar base and a 32-bit lane offset
+// instead of 64-bit flat addresses.
+typedef MGI_GLOBAL_AS double gdouble;
 
 __device__ __forceinline__ double wave_max_d(double v)
 {
@@ -122,7 +121,7 @@ struct TileMap {
 // latency is hidden behind the FMAs.
 template <bool HAS1>
 __device__ __forceinline__ void panel_columns(double (&x0)[16], double (&x1)[16], int& pos0, int& pos1,
-                                              bool& alive0, bool& alive1, int& info_local, int* __restrict__ ipiv,
+                                              bool& alive0, bool& alive1, int& info_local, MGI_GLOBAL_AS int* __restrict__ ipiv,
                                               int k0, int n, int l)
 {
     // idamax candidates of column c: NaN never wins, except at the first
@@ -211,9 +210,9 @@ template <int NT, bool WITH_B>
 struct TileCtx {
     TileSmem<NT, WITH_B>& sm;
     int n, nrhs, ldda, lddb;
-    double* A;
-    int* ipiv;
-    double* B;
+    MGI_GLOBAL_AS double* A;
+    MGI_GLOBAL_AS int* ipiv;
+    MGI_GLOBAL_AS double* B;
     int l, lr, lq;
 };
 
@@ -233,7 +232,7 @@ __device__ __forceinline__ void tile_wave_main(const TileCtx<NT, WITH_B>& cx)
     constexpr int SLD = TileSmem<NT, WITH_B>::SLD;
     auto& sm = cx.sm;
     const int n = cx.n, ldda = cx.ldda, lr = cx.lr, lq = cx.lq;
-    double* A = cx.A;
+    auto A = cx.A;
 
     // A tile is addressed as a wave-uniform base plus one of four 32-bit lane
     // offsets shared by all tiles (ldda < 2^27 keeps them in range).
@@ -245,7 +244,7 @@ __device__ __forceinline__ void tile_wave_main(const TileCtx<NT, WITH_B>& cx)
     for (int k = 0; k < KT; ++k) {
         if (!TileMap<NT>::valid(w, k)) continue;
         const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
-        const gdouble* At = as_global(A) + 16 * I + (size_t)(16 * J) * ldda;
+        const gdouble* At = A + 16 * I + (size_t)(16 * J) * ldda;
         if (16 * I + 16 <= n && 16 * J + 16 <= n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[k][r] = At[off[r]];
@@ -388,7 +387,7 @@ __device__ __forceinline__ void tile_wave_main(const TileCtx<NT, WITH_B>& cx)
     for (int k = 0; k < KT; ++k) {
         if (!TileMap<NT>::valid(w, k)) continue;
         const int I = TileMap<NT>::row(w, k), J = TileMap<NT>::col(w, k);
-        gdouble* At = as_global(A) + 16 * I + (size_t)(16 * J) * ldda;
+        gdouble* At = A + 16 * I + (size_t)(16 * J) * ldda;
         if (16 * I + 16 <= n && 16 * J + 16 <= n) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) At[off[r]] = acc[k][r];
@@ -448,7 +447,7 @@ __device__ __forceinline__ void panel_wave_main(const TileCtx<NT, WITH_B>& cx)
     constexpr int ULD = TileSmem<NT, WITH_B>::ULD;
     auto& sm = cx.sm;
     const int n = cx.n, l = cx.l, lr = cx.lr, lq = cx.lq;
-    int* ipiv = cx.ipiv;
+    auto ipiv = cx.ipiv;
 
     if constexpr (WITH_B) {
         for (int idx = l; idx < NP * 16; idx += kWave) {
@@ -675,7 +674,7 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
                            ta.nrhs_arr ? ta.nrhs_arr[mat] : ta.nrhs,
                            ta.ldda_arr ? ta.ldda_arr[mat] : ta.ldda,
                            ta.lddb_arr ? ta.lddb_arr[mat] : ta.lddb,
-                           dA_array[mat], ipiv_array[mat], WITH_B ? dB_array[mat] : nullptr,
+                           gptr(dA_array[mat]), gptr(ipiv_array[mat]), gptr(WITH_B ? dB_array[mat] : nullptr),
                            l, l % 16, l / 16};
     if (cx.n <= 0) return;                // empty system: nothing to factor
     switch (w) {
@@ -687,7 +686,7 @@ getrf_tile_kernel(TileArgs ta, double** __restrict__ dA_array, int** __restrict_
     __syncthreads();                      // sm.info final
     if (t == 0) info_array[mat] = sm.info;
     if constexpr (WITH_B) {
-        double* B = cx.B;
+        auto B = cx.B;
         for (int idx = t; idx < cx.n * cx.nrhs; idx += kTileThreads) {
             const int row = idx % cx.n, c = idx / cx.n;
             B[row + (size_t)c * cx.lddb] = sm.B[row * LD16 + c];

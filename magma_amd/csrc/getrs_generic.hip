@@ -23,6 +23,95 @@ namespace mgi {
 constexpr int kRsThreads = 256;
 constexpr int kRsCols = 8;     // right-hand sides per workgroup (16 when the batch has more than 8)
 
+// Q columns of one 64-row diagonal block of the unit lower factor, solved
+// as interleaved chains by one wave: lane l holds row l of the block
+// (lrow = its strictly lower part) and element l of columns col,
+// col + cstride, ... Row j's solved value reaches the rows below through a
+// cross-lane read.
+template <typename T, int Q>
+__device__ __forceinline__ void diag_lower(const T (&lrow)[kWave], T* col, int cstride, bool in)
+{
+    const int l = threadIdx.x % kWave;
+    T x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = in ? col[q * cstride] : T(0);
+#pragma unroll
+    for (int j = 0; j < kWave - 1; ++j) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const T xj = readlane_t(x[q], j);
+            if (l > j) x[q] = fma(-lrow[j], xj, x[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (in) col[q * cstride] = x[q];
+}
+
+// The same for the non-unit upper factor, backward: urow = the strictly upper
+// part of row l, rd = 1 / u_ll, nbk = rows in the block.
+template <typename T, int Q>
+__device__ __forceinline__ void diag_upper(const T (&urow)[kWave], T rd, int nbk, T* col, int cstride, bool in)
+{
+    const int l = threadIdx.x % kWave;
+    T x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = in ? col[q * cstride] : T(0);
+#pragma unroll
+    for (int j = kWave - 1; j >= 0; --j) {
+        if (j < nbk) {
+            const T dj = readlane_t(rd, j);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const T xj = readlane_t(x[q], j) * dj;
+                if (l == j) x[q] = xj;
+                else if (l < j) x[q] = fma(-urow[j], xj, x[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (in) col[q * cstride] = x[q];
+}
+
+// Rows [i0, i1) of the NC right-hand sides in sB (ld n) minus the 64-column
+// block kb of the factor times the block's solved rows: the unit lower
+// factor's block in ascending column order (UP false) or the upper factor's
+// in descending order (UP true), the order of the column-oriented sweep. One
+// row per thread; NC is the column count rounded up to 4 (the extra columns
+// hold zeros and are not stored).
+template <typename T, int NC, bool UP>
+__device__ __forceinline__ void apply_block(const MGI_GLOBAL_AS T* __restrict__ A, int ldda, T* sB, int n, int kb, int i0, int i1,
+                                            int nr)
+{
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += kRsThreads) {
+        T acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] = sB[i + c * n];
+        auto Ai = A + i + (size_t)kb * ldda;
+        // the row's factor entries are loaded CH at a time, each chunk
+        // whole before its first use
+        constexpr int CH = 32;
+        for (int j0 = 0; j0 < kWave; j0 += CH) {
+            T a[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int j = UP ? kWave - 1 - (j0 + u) : j0 + u;
+                a[u] = Ai[(size_t)j * ldda];
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int j = UP ? kWave - 1 - (j0 + u) : j0 + u;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[c] = fma(-a[u], sB[kb + j + c * n], acc[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c < nr) sB[i + c * n] = acc[c];
+    }
+}
+
 template <typename T, int NCOLS>
 __global__ void __launch_bounds__(kRsThreads)
 getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
@@ -38,9 +127,9 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     int* perm = reinterpret_cast<int*>(rs_raw + (size_t)sz.max_n * NCOLS * sizeof(T));
     const int tid = threadIdx.x;
     const int nr = min(NCOLS, nrhs - c0);
-    const T* A = dA_array[b];
-    const int* ipiv = ipiv_array ? ipiv_array[b] : nullptr;   // null: factors without pivoting
-    T* B = dB_array[b] + (size_t)c0 * lddb;
+    auto A = gptr(dA_array[b]);
+    auto ipiv = gptr(ipiv_array ? ipiv_array[b] : nullptr);   // null: factors without pivoting
+    auto B = gptr(dB_array[b]) + (size_t)c0 * lddb;
 
     // (P^T B)(i,:) = B(perm[i],:), P the product of the interchanges
     // k <-> ipiv[k]-1 in order. Each row is traced back through the
@@ -64,49 +153,45 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     __syncthreads();
 
     if (trans == 0) {
-        for (int idx = tid; idx < n * nr; idx += kRsThreads) {
+        // columns nr .. nc4 * 4 - 1 are zero: the block updates below run on
+        // whole groups of 4 columns
+        const int nc4 = (nr + 3) / 4;
+        for (int idx = tid; idx < n * nc4 * 4; idx += kRsThreads) {
             int i = idx % n, c = idx / n;
-            sB[i + c * n] = B[perm[i] + (size_t)c * lddb];
+            sB[i + c * n] = c < nr ? B[perm[i] + (size_t)c * lddb] : T(0);
         }
         __syncthreads();
         const int l = tid % kWave, w = tid / kWave;
         constexpr int kWaves = kRsThreads / kWave;
+        constexpr int kChains = NCOLS / kWaves;            // columns per wave in the diagonal-block solves
         for (int kb = 0; kb < n; kb += kWave) {            // unit lower, forward
             const int nbk = min(kWave, n - kb);
             const int r = kb + l;
             if (w < nr) {
-                // lane l holds row r of the diagonal block (strictly lower part)
+                // lane l holds row r of the diagonal block (strictly lower
+                // part); wave w solves columns w, w + 4, ... as interleaved
+                // chains
                 T lrow[kWave];
-                const T* Lc = A + r + (size_t)kb * ldda;
+                auto Lc = A + r + (size_t)kb * ldda;
 #pragma unroll
                 for (int j = 0; j < kWave; ++j) lrow[j] = (l > j && l < nbk) ? Lc[(size_t)j * ldda] : T(0);
-                for (int c = w; c < nr; c += kWaves) {
-                    T x = l < nbk ? sB[r + c * n] : T(0);
-#pragma unroll
-                    for (int j = 0; j < kWave - 1; ++j) {
-                        const T xj = readlane_t(x, j);
-                        if (l > j) x = fma(-lrow[j], xj, x);
-                    }
-                    if (l < nbk) sB[r + c * n] = x;
+                const int nq = (nr - w + kWaves - 1) / kWaves;
+                T* col = sB + r + w * n;
+                const bool in = l < nbk;
+                switch (nq) {
+                    case 1: diag_lower<T, 1>(lrow, col, kWaves * n, in); break;
+                    case 2: diag_lower<T, 2>(lrow, col, kWaves * n, in); break;
+                    case 3: diag_lower<T, 3>(lrow, col, kWaves * n, in); break;
+                    default: diag_lower<T, kChains>(lrow, col, kWaves * n, in); break;
                 }
             }
             __syncthreads();
             // rows below exist only after a full block
-            for (int i = kb + kWave + tid; i < n; i += kRsThreads) {
-                T acc[NCOLS];
-#pragma unroll
-                for (int c = 0; c < NCOLS; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
-                const T* Li = A + i + (size_t)kb * ldda;
-#pragma unroll 16
-                for (int j = 0; j < kWave; ++j) {
-                    const T lij = Li[(size_t)j * ldda];
-#pragma unroll
-                    for (int c = 0; c < NCOLS; ++c)
-                        if (c < nr) acc[c] = fma(-lij, sB[kb + j + c * n], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < NCOLS; ++c)
-                    if (c < nr) sB[i + c * n] = acc[c];
+            switch (nc4) {
+                case 1: apply_block<T, 4, false>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                case 2: apply_block<T, 8, false>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                case 3: apply_block<T, 12, false>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                default: apply_block<T, NCOLS, false>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
             }
             __syncthreads();
         }
@@ -115,42 +200,32 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
             const int nbk = kend - kb;
             const int r = kb + l;
             if (w < nr) {
-                // lane l holds row r of the diagonal block (strictly upper part)
+                // lane l holds row r of the diagonal block (strictly upper
+                // part); columns w, w + 4, ... run as interleaved chains
                 T urow[kWave];
-                const T* Uc = A + r + (size_t)kb * ldda;
+                auto Uc = A + r + (size_t)kb * ldda;
 #pragma unroll
                 for (int j = 0; j < kWave; ++j) urow[j] = (l < j && j < nbk) ? Uc[(size_t)j * ldda] : T(0);
-                const T ud = l < nbk ? A[r + (size_t)r * ldda] : T(1);
-                for (int c = w; c < nr; c += kWaves) {
-                    T x = l < nbk ? sB[r + c * n] : T(0);
-#pragma unroll
-                    for (int j = kWave - 1; j >= 0; --j) {
-                        if (j < nbk) {
-                            const T xj = readlane_t(x, j) / readlane_t(ud, j);
-                            if (l == j) x = xj;
-                            else if (l < j) x = fma(-urow[j], xj, x);
-                        }
-                    }
-                    if (l < nbk) sB[r + c * n] = x;
+                // each lane's reciprocal pivot, formed in parallel, keeps the
+                // division off the dependent chain
+                const T rd = l < nbk ? T(1) / A[r + (size_t)r * ldda] : T(1);
+                const int nq = (nr - w + kWaves - 1) / kWaves;
+                T* col = sB + r + w * n;
+                const bool in = l < nbk;
+                switch (nq) {
+                    case 1: diag_upper<T, 1>(urow, rd, nbk, col, kWaves * n, in); break;
+                    case 2: diag_upper<T, 2>(urow, rd, nbk, col, kWaves * n, in); break;
+                    case 3: diag_upper<T, 3>(urow, rd, nbk, col, kWaves * n, in); break;
+                    default: diag_upper<T, kChains>(urow, rd, nbk, col, kWaves * n, in); break;
                 }
             }
             __syncthreads();
             // rows above exist only above a full block
-            for (int i = tid; i < kb; i += kRsThreads) {
-                T acc[NCOLS];
-#pragma unroll
-                for (int c = 0; c < NCOLS; ++c) acc[c] = c < nr ? sB[i + c * n] : T(0);
-                const T* Ui = A + i + (size_t)kb * ldda;
-#pragma unroll 16
-                for (int j = kWave - 1; j >= 0; --j) {
-                    const T uij = Ui[(size_t)j * ldda];
-#pragma unroll
-                    for (int c = 0; c < NCOLS; ++c)
-                        if (c < nr) acc[c] = fma(-uij, sB[kb + j + c * n], acc[c]);
-                }
-#pragma unroll
-                for (int c = 0; c < NCOLS; ++c)
-                    if (c < nr) sB[i + c * n] = acc[c];
+            switch (nc4) {
+                case 1: apply_block<T, 4, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                case 2: apply_block<T, 8, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                case 3: apply_block<T, 12, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                default: apply_block<T, NCOLS, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
             }
             __syncthreads();
         }
@@ -216,9 +291,9 @@ getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, i
     if (n <= 0 || c0 >= nrhs) return;
     const int nr = min(kWaveCols, nrhs - c0);
     const int l = threadIdx.x % kWave;
-    const T* A = dA_array[b];
-    const int* ipiv = ipiv_array ? ipiv_array[b] : nullptr;   // null: factors without pivoting
-    T* B = dB_array[b] + (size_t)c0 * lddb;
+    auto A = gptr(dA_array[b]);
+    auto ipiv = gptr(ipiv_array ? ipiv_array[b] : nullptr);   // null: factors without pivoting
+    auto B = gptr(dB_array[b]) + (size_t)c0 * lddb;
 
     // (P^T B)(i,:) = B(src(i),:)
     int src[R];
@@ -251,7 +326,7 @@ getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, i
             for (int j = 0; j < CH; ++j) {
                 const int k = 64 * s + kk0 + j;
                 const bool kin = kk0 + j < kend;
-                const T* Lk = A + (size_t)(kin ? k : 0) * ldda;
+                auto Lk = A + (size_t)(kin ? k : 0) * ldda;
 #pragma unroll
                 for (int s2 = s; s2 < R; ++s2) {
                     const int i = l + 64 * s2;
@@ -296,7 +371,7 @@ getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, i
                 const int kk = kk0 - j;
                 const bool kin = kk >= 0;
                 const int k = 64 * s + (kin ? kk : 0);
-                const T* Uk = A + (size_t)k * ldda;
+                auto Uk = A + (size_t)k * ldda;
                 dg[j] = Uk[k];
 #pragma unroll
                 for (int s2 = 0; s2 <= s; ++s2) {

@@ -37,6 +37,20 @@ void hip_report(hipError_t err, const char* what, const char* file, int line);
 // into launches of at most this many matrices.
 constexpr int kMaxGridYZ = 65535;
 
+// A matrix pointer read from a device array of pointers carries no address
+// space, so accesses through it compile to flat instructions, which count
+// against the LDS/scalar wait counter as well: every wait for an LDS read
+// then also waits for the outstanding global loads. The batch's matrices
+// always live in global memory; gptr() states that to the compiler.
+#ifndef MGI_GLOBAL_AS
+#define MGI_GLOBAL_AS __attribute__((address_space(1)))
+#endif
+template <typename T>
+__device__ __forceinline__ MGI_GLOBAL_AS T* gptr(T* p)
+{
+    return (MGI_GLOBAL_AS T*)p;
+}
+
 }  // namespace mgi
 
 #define MGI_HIP_CHECK(call)                                                    \

@@ -18,6 +18,7 @@
 #include "getrf_kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -112,13 +113,30 @@ static bool build_buckets(const int* h_m, const int* h_n, const int* h_nrhs, int
 }
 
 // Launch every bucket (asynchronous, allocation-free). d_map holds the
-// concatenated member lists; m_arr may be null for square systems.
+// concatenated member lists; m_arr may be null for square systems. Bucket i
+// goes to streams[lane[i]] when lanes are given (the buckets touch disjoint
+// systems and share no scratch, so they may run concurrently), else to
+// stream.
+constexpr int kVbLanes = 8;          // most concurrent streams of a captured plan
+constexpr int kVbLanesDefault = 4;
+constexpr int kVbSplitDefault = 4;   // pieces per blocked bucket
+
+// integer from the environment, or fallback when unset
+static int env_int(const char* name, int fallback)
+{
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : fallback;
+}
+
 static magma_int_t exec_buckets(const std::vector<VBucket>& buckets, const int* d_map, const int* m_arr,
                                 const int* n_arr, const int* nrhs_arr, double** dA_array, const int* ldda_arr,
                                 int** ipiv_array, double** dB_array, const int* lddb_arr, int* info_array,
-                                bool solve, hipStream_t stream)
+                                bool solve, hipStream_t stream, const hipStream_t* streams = nullptr,
+                                const int* lane = nullptr)
 {
-    for (const VBucket& vb : buckets) {
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        const VBucket& vb = buckets[bi];
+        if (streams && lane) stream = streams[lane[bi]];
         const int* map = d_map + vb.offset;
         magma_int_t rc;
         if (vb.variant == kVbTile) {
@@ -407,11 +425,64 @@ magma_int_t magma_dgesv_vbatched_plan_capture(magma_vbatched_plan_t plan, magma_
     }
     magma_int_t rc = MAGMA_SUCCESS;
     vb_fill_kernel<<<ceildiv(plan->batch, 256), 256, 0, s>>>(dinfo_array, 0, plan->batch);
-    if (!plan->buckets.empty())
-        rc = exec_buckets(plan->buckets, plan->d_map, nullptr, n, plan->solve ? nrhs : nullptr, dA_array, ldda,
-                          dipiv_array, dB_array, lddb, dinfo_array, plan->solve, s);
+    // The buckets are independent: they are recorded on up to kVbLanes
+    // forked streams (greedy balance of count * n^3), so the graph lets a
+    // small bucket run beside a large one; a blocked bucket is recorded as
+    // `split` pieces of consecutive members so the panel phase of one piece
+    // overlaps the update of another. The lanes join back into s.
+    const int nlanes = std::max(1, std::min(kVbLanes, env_int("MAGMA_VB_LANES", kVbLanesDefault)));
+    const int split = std::max(1, std::min(16, env_int("MAGMA_VB_SPLIT", kVbSplitDefault)));
+    std::vector<VBucket> pieces;
+    for (const VBucket& vb : plan->buckets) {
+        const int np = vb.variant == kVbTile ? 1 : std::min(split, vb.count);
+        for (int p = 0; p < np; ++p) {
+            VBucket pc = vb;
+            const int lo = (int)((long long)vb.count * p / np), hi = (int)((long long)vb.count * (p + 1) / np);
+            pc.offset = vb.offset + lo;
+            pc.count = hi - lo;
+            if (pc.count > 0) pieces.push_back(pc);
+        }
+    }
+    const size_t npc = pieces.size();
+    std::vector<int> lane(npc, 0);
+    {
+        std::vector<double> load(nlanes, 0.0);
+        std::vector<size_t> order(npc);
+        for (size_t i = 0; i < npc; ++i) order[i] = i;
+        auto cost = [&](size_t i) {
+            const double m = pieces[i].max_n;
+            return (double)pieces[i].count * m * m * m;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(x) > cost(y); });
+        for (size_t i : order) {
+            const int best = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane[i] = best;
+            load[best] += cost(i);
+        }
+    }
+    hipStream_t ls[kVbLanes] = {};
+    hipEvent_t fork = nullptr, joins[kVbLanes] = {};
+    bool lanes_ok = hipEventCreateWithFlags(&fork, hipEventDisableTiming) == hipSuccess &&
+                    hipEventRecord(fork, s) == hipSuccess;
+    for (int q = 0; q < nlanes && lanes_ok; ++q) {
+        lanes_ok = hipStreamCreateWithFlags(&ls[q], hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&joins[q], hipEventDisableTiming) == hipSuccess &&
+                   hipStreamWaitEvent(ls[q], fork, 0) == hipSuccess;
+    }
+    if (!lanes_ok) rc = MAGMA_ERR_UNKNOWN;
+    if (rc == MAGMA_SUCCESS && npc > 0)
+        rc = exec_buckets(pieces, plan->d_map, nullptr, n, plan->solve ? nrhs : nullptr, dA_array, ldda,
+                          dipiv_array, dB_array, lddb, dinfo_array, plan->solve, s, ls, lane.data());
+    for (int q = 0; q < nlanes; ++q) {
+        if (ls[q] && joins[q] && hipEventRecord(joins[q], ls[q]) == hipSuccess) (void)hipStreamWaitEvent(s, joins[q], 0);
+    }
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
+    for (int q = 0; q < nlanes; ++q) {
+        if (joins[q]) hipEventDestroy(joins[q]);
+        if (ls[q]) hipStreamDestroy(ls[q]);
+    }
+    if (fork) hipEventDestroy(fork);
     hipStreamDestroy(s);
     if (rc != MAGMA_SUCCESS) { if (g) hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) { hip_report(e, "hipStreamEndCapture", __FILE__, __LINE__); return MAGMA_ERR_UNKNOWN; }

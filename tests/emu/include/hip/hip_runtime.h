@@ -96,6 +96,8 @@ inline void __syncthreads() { emu::block_sync(); }
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_sched_barrier(...) ((void)0)
+#define __builtin_amdgcn_s_waitcnt(...) ((void)0)
+#define MGI_GLOBAL_AS
 
 template <class T>
 inline T emu_shfl(T v, int src)

@@ -63,7 +63,15 @@ def test_strerror_and_nbparam(lib):
     assert lib.magma_strerror(-3) == b"invalid argument"
     nb, rec = ctypes.c_int(), ctypes.c_int()
     lib.magma_get_dgetrf_batched_nbparam(128, ctypes.byref(nb), ctypes.byref(rec))
-    assert nb.value > 0 and rec.value > 0
+    assert (nb.value, rec.value) == (16, 16)            # register-tile LU: 16-column panels
+    lib.magma_get_dgetrf_batched_nbparam(400, ctypes.byref(nb), ctypes.byref(rec))
+    assert (nb.value, rec.value) == (64, 32)            # blocked LU: 64-wide blocks of 32-wide panels
+    lib.magma_get_dgetrf_batched_nbparam(1000, ctypes.byref(nb), ctypes.byref(rec))
+    assert (nb.value, rec.value) == (16, 16)            # 4 rows per thread: 16-wide panels
+    lib.magma_get_dgetrf_vbatched_nbparam(100, 100, ctypes.byref(nb), ctypes.byref(rec))
+    assert (nb.value, rec.value) == (16, 16)
+    lib.magma_get_dgetrf_vbatched_nbparam(200, 200, ctypes.byref(nb), ctypes.byref(rec))
+    assert (nb.value, rec.value) == (32, 32)
 
 
 def test_xerbla_message(lib, capfd):
