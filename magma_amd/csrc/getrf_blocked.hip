@@ -205,6 +205,7 @@ constexpr int kLaswpCols = 32;     // columns per workgroup
 // 32-wide panels): see blocked_steps2.
 constexpr int kInPanelRight = 1;
 constexpr int kInPanelLeft = 2;
+constexpr int kRightOnly = 3;     // only the columns right of the panel
 
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -217,7 +218,9 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(nb, minmn - k);
-    // mode 0: every column outside the panel; kInPanelRight: the columns of
+    // mode 0: every column outside the panel; kRightOnly: the columns right
+    // of the panel (the left ones get the deferred pass of
+    // laswp_deferred_kernel); kInPanelRight: the columns of
     // the second half of a 64-wide block (panel = its first half); kInPanelLeft:
     // the 32 columns left of the panel (panel = the block's second half)
     int span = n - jb, lo = 0;
@@ -229,6 +232,9 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
     } else if (mode == kInPanelLeft) {
         lo = k - nb;
         span = nb;
+    } else if (mode == kRightOnly) {
+        lo = k + jb;
+        span = n - k - jb;
     }
     const int cbeg = blockIdx.x * kLaswpCols;
     if (cbeg >= span) return;
@@ -289,6 +295,89 @@ laswp_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int** __rest
 }
 
 // ---------------------------------------------------------------------------
+// 2b. Deferred interchanges of the factored columns. The L columns of block
+//     bk (columns bk*nbk .. bk*nbk + nbk - 1) are read by no later step, so
+//     the interchanges of the later steps (rows t >= T0 = (bk+1)*nbk) are
+//     applied to them once, after the last step, instead of one scattered
+//     pass per step. Every row r >= T0 is traced back through those
+//     interchanges (last to first) to the row whose content ends there, as
+//     in laswp_kernel; then each column moves its rows, all loads of a batch
+//     of columns completing before any store. The composition equals applying
+//     the interchanges in order, so the factors are identical to those of a
+//     step-by-step dlaswp.
+// ---------------------------------------------------------------------------
+constexpr int kDeferMaxSteps = 2048;   // later steps one workgroup traces (max_m of the blocked LU)
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256)
+laswp_deferred_kernel(SizeArgs sz, int nbk, T** __restrict__ dA_array, int** __restrict__ ipiv_array)
+{
+    __shared__ int s_piv[kDeferMaxSteps];
+    const int b = sz.mat(blockIdx.y);
+    int m, n, ld;
+    sz.get(b, m, n, ld);
+    const int minmn = min(m, n);
+    const int c0 = blockIdx.x * nbk;
+    const int T0 = c0 + nbk;
+    if (T0 >= minmn) return;                     // no later step for this block
+    const int ncols = nbk;                       // c0 + nbk = T0 < minmn <= n
+    const int tid = threadIdx.x;
+    auto ip = gptr(ipiv_array[b]);
+    for (int t = T0 + tid; t < minmn; t += 256) {
+        const int p = ip[t] - 1;
+        s_piv[t - T0] = (p > t && p < m) ? p : t;
+    }
+    __syncthreads();
+    int src[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) src[i] = T0 + tid + 256 * i;
+    for (int t = minmn - 1; t >= T0; --t) {
+        const int p = s_piv[t - T0];
+#pragma unroll
+        for (int i = 0; i < R; ++i) src[i] = (src[i] == t) ? p : (src[i] == p ? t : src[i]);
+    }
+    bool mv[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) mv[i] = T0 + tid + 256 * i < m && src[i] != T0 + tid + 256 * i;
+    auto A = gptr(dA_array[b]) + (size_t)c0 * ld;
+    constexpr int CB = R <= 2 ? 8 : (R <= 4 ? 4 : 2);   // columns per batch
+    for (int cb = 0; cb < ncols; cb += CB) {
+        T v[CB][R];
+#pragma unroll
+        for (int u = 0; u < CB; ++u)
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                v[u][i] = (cb + u < ncols && mv[i]) ? A[(size_t)(cb + u) * ld + src[i]] : T(0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);       // vmcnt(0): every load has returned
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < CB; ++u)
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (cb + u < ncols && mv[i]) A[(size_t)(cb + u) * ld + T0 + tid + 256 * i] = v[u][i];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+static magma_int_t deferred_launch(int R, const SizeArgs& sz, int nbk, int max_minmn, T** dA_array,
+                                   int** ipiv_array, int batch, hipStream_t stream)
+{
+    const int nblk = ceildiv(max_minmn, nbk) - 1;   // blocks followed by a later step
+    if (ipiv_array == nullptr || nblk <= 0) return MAGMA_SUCCESS;
+    const dim3 grid(nblk, batch);
+    switch (R) {
+        case 1: laswp_deferred_kernel<T, 1><<<grid, 256, 0, stream>>>(sz, nbk, dA_array, ipiv_array); break;
+        case 2: laswp_deferred_kernel<T, 2><<<grid, 256, 0, stream>>>(sz, nbk, dA_array, ipiv_array); break;
+        case 4: laswp_deferred_kernel<T, 4><<<grid, 256, 0, stream>>>(sz, nbk, dA_array, ipiv_array); break;
+        case 8: laswp_deferred_kernel<T, 8><<<grid, 256, 0, stream>>>(sz, nbk, dA_array, ipiv_array); break;
+        default: return MAGMA_ERR_NOT_SUPPORTED;
+    }
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
 // 3. A12 := L11^{-1} A12 (unit lower). NB lanes run down the rows of one
 //    column (64/NB columns per wave instruction), so loads and stores are
 //    contiguous; the solved x_j reaches the rows below through scalar lane
@@ -328,23 +417,38 @@ trsm_kernel(SizeArgs sz, int k, T** __restrict__ dA_array, int mode)
     const int l = t % kWave, w = t / kWave;
     const int i = l % NB, cq = l / NB;
     const int cend = min(ncols, c0 + kTrsmCols);
-    for (int cb = c0 + w * CPW; cb < cend; cb += (256 / kWave) * CPW) {
-        const int c = cb + cq;
-        const bool on = c < cend && i < jb;
-        auto X = gptr(dA_array[b]) + k + (size_t)(k + jb + c) * ld;
-        T x = on ? X[i] : T(0);
+    // Q column groups per wave pass: their loads are in flight together and
+    // their chains interleave (each column's operation order is unchanged)
+    constexpr int Q = 4;
+    constexpr int kStride = (256 / kWave) * CPW;    // columns between a wave's groups
+    auto A0 = gptr(dA_array[b]) + k + (size_t)(k + jb) * ld;
+    for (int cb = c0 + w * CPW; cb < cend; cb += kStride * Q) {
+        T x[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int c = cb + q * kStride + cq;
+            x[q] = (c < cend && i < jb) ? A0[(size_t)c * ld + i] : T(0);
+        }
 #pragma unroll
         for (int j = 0; j < NB - 1; ++j) {
-            // x_j of this lane's column: one scalar read per column group
-            T xj = readlane_t(x, j);
+            const T lij = sL[i + j * NB];
 #pragma unroll
-            for (int g = 1; g < CPW; ++g) {
-                const T v = readlane_t(x, g * NB + j);
-                xj = cq == g ? v : xj;
+            for (int q = 0; q < Q; ++q) {
+                // x_j of this lane's column: one scalar read per column group
+                T xj = readlane_t(x[q], j);
+#pragma unroll
+                for (int g = 1; g < CPW; ++g) {
+                    const T v = readlane_t(x[q], g * NB + j);
+                    xj = cq == g ? v : xj;
+                }
+                if (i > j) x[q] = fma(-lij, xj, x[q]);
             }
-            if (i > j) x = fma(-sL[i + j * NB], xj, x);
         }
-        if (on) X[i] = x;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int c = cb + q * kStride + cq;
+            if (c < cend && i < jb) A0[(size_t)c * ld + i] = x[q];
+        }
     }
 }
 
@@ -723,10 +827,9 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
     for (int k = 0; k < max_minmn; k += NB) {
         magma_int_t prc = panel_launch<T, NB>(R, sz, k, dA_array, ipiv_array, info_array, batch, stream);
         if (prc != MAGMA_SUCCESS) return prc;
-        const int other = max_n;   // upper bound of the columns outside the panel
-        if (ipiv_array) {
-            laswp_kernel<T><<<dim3(ceildiv(other, kLaswpCols), batch), 256, 0, stream>>>(sz, k, NB, dA_array,
-                                                                                       ipiv_array, 0);
+        if (ipiv_array && max_n - k - 1 > 0) {        // columns right of the panel
+            laswp_kernel<T><<<dim3(ceildiv(max_n - k - 1, kLaswpCols), batch), 256, 0, stream>>>(
+                sz, k, NB, dA_array, ipiv_array, kRightOnly);
             MGI_LAUNCH_CHECK();
         }
         const int rest_n = max_n - k - 1;
@@ -740,7 +843,8 @@ static magma_int_t blocked_steps(int R, const SizeArgs& sz, int max_m, int max_n
             }
         }
     }
-    return MAGMA_SUCCESS;
+    // the interchanges of later steps on the factored columns
+    return deferred_launch<T>(R, sz, NB, max_minmn, dA_array, ipiv_array, batch, stream);
 }
 
 // Two-level blocking: each 64-wide block is factored as two 32-wide panels
@@ -781,9 +885,9 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
             }
         }
         // the whole block's interchanges outside it, U12 and the trailing update
-        if (ipiv_array) {
-            laswp_kernel<T><<<dim3(ceildiv(max_n, kLaswpCols), batch), 256, 0, stream>>>(sz, k, 2 * H, dA_array,
-                                                                                       ipiv_array, 0);
+        if (ipiv_array && max_n - k - 1 > 0) {        // columns right of the block
+            laswp_kernel<T><<<dim3(ceildiv(max_n - k - 1, kLaswpCols), batch), 256, 0, stream>>>(
+                sz, k, 2 * H, dA_array, ipiv_array, kRightOnly);
             MGI_LAUNCH_CHECK();
         }
         const int rest_n = max_n - k - 1;
@@ -798,7 +902,8 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
             }
         }
     }
-    return MAGMA_SUCCESS;
+    // the interchanges of later blocks on the factored columns
+    return deferred_launch<T>(R, sz, 2 * H, max_minmn, dA_array, ipiv_array, batch, stream);
 }
 
 bool getrf_blocked_supported(int max_m) { BlockedPlan p; return plan_for(max_m, p); }
