@@ -692,17 +692,24 @@ static magma_int_t panel_launch(int R, const SizeArgs& sz, int k, T** dA_array, 
     return MAGMA_SUCCESS;
 }
 
-// The full trailing update of larger systems (FP64 or FP32): 128 x 128
-// output tiles, so each A21 / U12 strip is read half as often as with the
-// 64 x 64 tiles above. Four waves of 64 x 64 (4 x 4 MFMA tiles each); K
-// staged through LDS 16 at a time, double-buffered, one barrier per stage;
-// the product is formed transposed (the same store pattern as above).
-// Grid: x = tile of the system (column-major over the tile grid), y = system.
-template <typename T>
+// The full trailing update in FP64 or FP32: TM x TM output tiles (TM = 128
+// for trailing matrices of at least 256 x 256, so each A21 / U12 strip is
+// read half as often; TM = 64 below that). Four waves of TM/2 x TM/2
+// (TM/32 x TM/32 MFMA tiles each); K staged through LDS 16 at a time,
+// double-buffered, one barrier per stage, so the next stage's loads overlap
+// the current products; the product is formed transposed (the same store
+// pattern as above). Grid: x = tile of the system (column-major over the
+// tile grid), y = system.
+template <typename T, int TM>
 __global__ void __launch_bounds__(256, 2)
-gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
+gemm_update_db_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
 {
-    constexpr int TM = 128, BK = 16, LDP = TM + 4;
+    constexpr int BK = 16, LDP = TM + 4;
+    constexpr int WT = TM / 2;              // each wave's square of the tile
+    constexpr int MT = WT / 16;             // MFMA tiles per wave and dimension
+    constexpr int AS = 256 / TM;            // k stride between a thread's A elements
+    constexpr int AQ = TM * BK / 256;       // A elements per thread and stage
+    constexpr int BQ = TM / 16;             // B elements per thread and stage
     const int b = sz.mat(blockIdx.y);
     int m, n, ld;
     sz.get(b, m, n, ld);
@@ -721,34 +728,36 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
     auto C = Ab + (k + jb) + (size_t)(k + jb) * ld;
     const int t = threadIdx.x;
     const int w = t / kWave, l = t % kWave;
-    const int wr = (w & 1) * 64, wc = (w >> 1) * 64;
+    const int wr = (w & 1) * WT, wc = (w >> 1) * WT;
     const int lr = l % 16, lq = l / 16;
     __shared__ T sA[2][BK][LDP];
     __shared__ T sB[2][BK][LDP];
     using acc_t = typename std::conditional<std::is_same<T, double>::value, double4_t, float4_t>::type;
-    acc_t acc[4][4];
+    acc_t acc[MT][MT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < MT; ++a)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
-    T ra[8], rb[8];
-    const int ai = t & 127, ak = t >> 7;       // A tile: row ai, k = ak + 2q
+        for (int c = 0; c < MT; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+    T ra[AQ], rb[BQ];
+    const int ai = t % TM, ak = t / TM;        // A tile: row ai, k = ak + AS q
     const int bk = t & 15, bj = t >> 4;        // B tile: k = bk, column bj + 16q
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int kk = ak + 2 * q;
+        for (int q = 0; q < AQ; ++q) {
+            const int kk = ak + AS * q;
             ra[q] = (m0 + ai < M && k0 + kk < K) ? A[(size_t)(m0 + ai) + (size_t)(k0 + kk) * ld] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
             const int jj = bj + 16 * q;
             rb[q] = (k0 + bk < K && n0 + jj < N) ? B[(size_t)(k0 + bk) + (size_t)(n0 + jj) * ld] : T(0);
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            sA[buf][ak + 2 * q][ai] = ra[q];
-            sB[buf][bk][bj + 16 * q] = rb[q];
-        }
+        for (int q = 0; q < AQ; ++q) sA[buf][ak + AS * q][ai] = ra[q];
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) sB[buf][bk][bj + 16 * q] = rb[q];
     };
     const int nst = (K + BK - 1) / BK;
     gload(0);
@@ -759,16 +768,16 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
         if (st + 1 < nst) gload((st + 1) * BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            T pf[4], qf[4];
+            T pf[MT], qf[MT];
 #pragma unroll
-            for (int x = 0; x < 4; ++x) {
+            for (int x = 0; x < MT; ++x) {
                 pf[x] = sB[buf][kk + lq][wc + 16 * x + lr];
                 qf[x] = sA[buf][kk + lq][wr + 16 * x + lr];
             }
 #pragma unroll
-            for (int tj = 0; tj < 4; ++tj)
+            for (int tj = 0; tj < MT; ++tj)
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti) {
+                for (int ti = 0; ti < MT; ++ti) {
                     if constexpr (std::is_same<T, double>::value)
                         acc[tj][ti] = __builtin_amdgcn_mfma_f64_16x16x4f64(pf[tj], qf[ti], acc[tj][ti], 0, 0, 0);
                     else
@@ -779,9 +788,9 @@ gemm_update128_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
         __syncthreads();
     }
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+    for (int tj = 0; tj < MT; ++tj)
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti) {
+        for (int ti = 0; ti < MT; ++ti) {
             const int row = m0 + wr + 16 * ti + lr;
             if (row >= M) continue;
 #pragma unroll
@@ -808,10 +817,14 @@ static magma_int_t trailing_update(const SizeArgs& sz, int k, int rest_m, int re
     if constexpr (MODE == kUpdF64 || MODE == kUpdF32) {
         if (rest_m >= 256 && rest_n >= 256) {
             const int tiles = ceildiv(rest_m, 128) * ceildiv(rest_n, 128);
-            gemm_update128_kernel<T><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
+            gemm_update_db_kernel<T, 128><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
             MGI_LAUNCH_CHECK();
             return MAGMA_SUCCESS;
         }
+        const int tiles = ceildiv(rest_m, 64) * ceildiv(rest_n, 64);
+        gemm_update_db_kernel<T, 64><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
+        MGI_LAUNCH_CHECK();
+        return MAGMA_SUCCESS;
     }
     dim3 grid(ceildiv(rest_m, 64), ceildiv(rest_n, 64), batch);
     gemm_update_kernel<T, NB, MODE><<<grid, 256, 0, stream>>>(sz, k, NB, dA_array, 0);
