@@ -75,12 +75,12 @@ __device__ __forceinline__ void diag_upper(const T (&urow)[kWave], T rd, int nbk
 }
 
 // Rows [i0, i1) of the NC right-hand sides in sB (ld n) minus the 64-column
-// block kb of the factor times the block's solved rows: the unit lower
-// factor's block in ascending column order (UP false) or the upper factor's
-// in descending order (UP true), the order of the column-oriented sweep. One
-// row per thread; NC is the column count rounded up to 4 (the extra columns
-// hold zeros and are not stored).
-template <typename T, int NC, bool UP>
+// block kb of the factor times the block's solved rows: the block's steps in
+// ascending order (UP false) or descending order (UP true), the order of the
+// column-oriented sweep. TR reads the factor transposed (the Trans solves'
+// U^T and L^T). One row per thread; NC is the column count rounded up to 4
+// (the extra columns hold zeros and are not stored).
+template <typename T, int NC, bool UP, bool TR = false>
 __device__ __forceinline__ void apply_block(const MGI_GLOBAL_AS T* __restrict__ A, int ldda, T* sB, int n, int kb, int i0, int i1,
                                             int nr)
 {
@@ -88,7 +88,10 @@ __device__ __forceinline__ void apply_block(const MGI_GLOBAL_AS T* __restrict__ 
         T acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c] = sB[i + c * n];
-        auto Ai = A + i + (size_t)kb * ldda;
+        // step j's factor entry of row i: A(i, kb + j), or A(kb + j, i) for
+        // the transposed factor (TR; one contiguous column segment per row)
+        const size_t sj = TR ? 1 : (size_t)ldda;
+        auto Ai = TR ? A + kb + (size_t)i * ldda : A + i + (size_t)kb * ldda;
         // the row's factor entries are loaded CH at a time, each chunk
         // whole before its first use
         constexpr int CH = 32;
@@ -97,7 +100,7 @@ __device__ __forceinline__ void apply_block(const MGI_GLOBAL_AS T* __restrict__ 
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
                 const int j = UP ? kWave - 1 - (j0 + u) : j0 + u;
-                a[u] = Ai[(size_t)j * ldda];
+                a[u] = Ai[(size_t)j * sj];
             }
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
@@ -112,9 +115,36 @@ __device__ __forceinline__ void apply_block(const MGI_GLOBAL_AS T* __restrict__ 
     }
 }
 
-template <typename T, int NCOLS>
+// Q columns of one 64-row diagonal block of a non-unit lower factor (U^T in
+// the Trans solve), forward, as interleaved chains: lrow = the strictly lower
+// part of row l, rd = 1 / diagonal of row l, nbk = rows in the block.
+template <typename T, int Q>
+__device__ __forceinline__ void diag_lower_nonunit(const T (&lrow)[kWave], T rd, int nbk, T* col, int cstride, bool in)
+{
+    const int l = threadIdx.x % kWave;
+    T x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = in ? col[q * cstride] : T(0);
+#pragma unroll
+    for (int j = 0; j < kWave; ++j) {
+        if (j < nbk) {
+            const T dj = readlane_t(rd, j);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const T xj = readlane_t(x[q], j) * dj;
+                if (l == j) x[q] = xj;
+                else if (l > j) x[q] = fma(-lrow[j], xj, x[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        if (in) col[q * cstride] = x[q];
+}
+
+template <typename T, int NCOLS, bool TRANS>
 __global__ void __launch_bounds__(kRsThreads)
-getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
+getrs_generic_kernel(RsSizes sz, T** __restrict__ dA_array, int** __restrict__ ipiv_array,
                      T** __restrict__ dB_array)
 {
     extern __shared__ __align__(16) unsigned char rs_raw[];
@@ -152,7 +182,7 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
     }
     __syncthreads();
 
-    if (trans == 0) {
+    if constexpr (!TRANS) {
         // columns nr .. nc4 * 4 - 1 are zero: the block updates below run on
         // whole groups of 4 columns
         const int nc4 = (nr + 3) / 4;
@@ -234,25 +264,75 @@ getrs_generic_kernel(int trans, RsSizes sz, T** __restrict__ dA_array, int** __r
             B[i + (size_t)c * lddb] = sB[i + c * n];
         }
     } else {
-        for (int idx = tid; idx < n * nr; idx += kRsThreads) {
+        // Blocked by 64 rows like the NoTrans sweeps: a wave per column group
+        // solves the diagonal block, then all threads apply the block to the
+        // remaining rows. Per element, the contributions arrive in the order
+        // of the column-oriented sweeps (ascending for U^T, descending for
+        // L^T).
+        const int nc4 = (nr + 3) / 4;
+        for (int idx = tid; idx < n * nc4 * 4; idx += kRsThreads) {
             int i = idx % n, c = idx / n;
-            sB[i + c * n] = B[i + (size_t)c * lddb];
+            sB[i + c * n] = c < nr ? B[i + (size_t)c * lddb] : T(0);
         }
         __syncthreads();
-        for (int k = 0; k < n; ++k) {                      // U^T: non-unit lower, forward
-            if (tid < nr) sB[k + tid * n] = sB[k + tid * n] / A[k + (size_t)k * ldda];
+        const int l = tid % kWave, w = tid / kWave;
+        constexpr int kWaves = kRsThreads / kWave;
+        constexpr int kChains = NCOLS / kWaves;
+        for (int kb = 0; kb < n; kb += kWave) {            // U^T: non-unit lower, forward
+            const int nbk = min(kWave, n - kb);
+            const int r = kb + l;
+            if (w < nr) {
+                // lane l: row r of U^T in the block, U(kb + j, r) for j < l
+                T lrow[kWave];
+                auto Uc = A + kb + (size_t)r * ldda;
+#pragma unroll
+                for (int j = 0; j < kWave; ++j) lrow[j] = (j < l && l < nbk) ? Uc[j] : T(0);
+                const T rd = l < nbk ? T(1) / A[r + (size_t)r * ldda] : T(1);
+                const int nq = (nr - w + kWaves - 1) / kWaves;
+                T* col = sB + r + w * n;
+                const bool in = l < nbk;
+                switch (nq) {
+                    case 1: diag_lower_nonunit<T, 1>(lrow, rd, nbk, col, kWaves * n, in); break;
+                    case 2: diag_lower_nonunit<T, 2>(lrow, rd, nbk, col, kWaves * n, in); break;
+                    case 3: diag_lower_nonunit<T, 3>(lrow, rd, nbk, col, kWaves * n, in); break;
+                    default: diag_lower_nonunit<T, kChains>(lrow, rd, nbk, col, kWaves * n, in); break;
+                }
+            }
             __syncthreads();
-            const int rows = n - k - 1;
-            for (int idx = tid; idx < rows * nr; idx += kRsThreads) {
-                int i = k + 1 + idx % rows, c = idx / rows;
-                sB[i + c * n] = fma(-A[k + (size_t)i * ldda], sB[k + c * n], sB[i + c * n]);
+            switch (nc4) {
+                case 1: apply_block<T, 4, false, true>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                case 2: apply_block<T, 8, false, true>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                case 3: apply_block<T, 12, false, true>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
+                default: apply_block<T, NCOLS, false, true>(A, ldda, sB, n, kb, kb + kWave, n, nr); break;
             }
             __syncthreads();
         }
-        for (int k = n - 1; k > 0; --k) {                  // L^T: unit upper, backward
-            for (int idx = tid; idx < k * nr; idx += kRsThreads) {
-                int i = idx % k, c = idx / k;
-                sB[i + c * n] = fma(-A[k + (size_t)i * ldda], sB[k + c * n], sB[i + c * n]);
+        for (int kend = n; kend > 0; kend -= kWave) {      // L^T: unit upper, backward
+            const int kb = max(0, kend - kWave);
+            const int nbk = kend - kb;
+            const int r = kb + l;
+            if (w < nr) {
+                // lane l: row r of L^T in the block, L(kb + j, r) for j > l
+                T urow[kWave];
+                auto Lc = A + kb + (size_t)r * ldda;
+#pragma unroll
+                for (int j = 0; j < kWave; ++j) urow[j] = (l < j && j < nbk) ? Lc[j] : T(0);
+                const int nq = (nr - w + kWaves - 1) / kWaves;
+                T* col = sB + r + w * n;
+                const bool in = l < nbk;
+                switch (nq) {                                  // unit diagonal: reciprocal 1
+                    case 1: diag_upper<T, 1>(urow, T(1), nbk, col, kWaves * n, in); break;
+                    case 2: diag_upper<T, 2>(urow, T(1), nbk, col, kWaves * n, in); break;
+                    case 3: diag_upper<T, 3>(urow, T(1), nbk, col, kWaves * n, in); break;
+                    default: diag_upper<T, kChains>(urow, T(1), nbk, col, kWaves * n, in); break;
+                }
+            }
+            __syncthreads();
+            switch (nc4) {
+                case 1: apply_block<T, 4, true, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                case 2: apply_block<T, 8, true, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                case 3: apply_block<T, 12, true, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
+                default: apply_block<T, NCOLS, true, true>(A, ldda, sB, n, kb, 0, kb, nr); break;
             }
             __syncthreads();
         }
@@ -432,6 +512,18 @@ static size_t generic_bytes(int n, int elem, int ncols) { return (size_t)n * nco
 size_t getrs_generic_bytes(int n, int elem) { return generic_bytes(n, elem, kRsCols); }
 size_t getrs_generic_bytes(int n) { return getrs_generic_bytes(n, sizeof(double)); }
 
+template <typename T, int NC, bool TRANS>
+static magma_int_t generic_launch(const RsSizes& sz, int max_nrhs, size_t lds, T** dA_array, int** ipiv_array,
+                                  T** dB_array, int batch, hipStream_t stream)
+{
+    MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T, NC, TRANS>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    dim3 grid(batch, ceildiv(max_nrhs, NC));
+    getrs_generic_kernel<T, NC, TRANS><<<grid, kRsThreads, lds, stream>>>(sz, dA_array, ipiv_array, dB_array);
+    MGI_LAUNCH_CHECK();
+    return MAGMA_SUCCESS;
+}
+
 template <typename T>
 magma_int_t getrs_generic_launch_t(int trans, const RsSizes& sz, int max_nrhs, T** dA_array, int** ipiv_array,
                                    T** dB_array, int batch, hipStream_t stream)
@@ -447,23 +539,13 @@ magma_int_t getrs_generic_launch_t(int trans, const RsSizes& sz, int max_nrhs, T
     // more than 8 right-hand sides: 16 per workgroup, so each system's
     // factors are streamed once for up to 16 columns
     const size_t lds16 = generic_bytes(sz.max_n, sizeof(T), 2 * kRsCols);
-    if (max_nrhs > kRsCols && lds16 <= max_dynamic_lds()) {
-        MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T, 2 * kRsCols>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds16));
-        dim3 grid(batch, ceildiv(max_nrhs, 2 * kRsCols));
-        getrs_generic_kernel<T, 2 * kRsCols><<<grid, kRsThreads, lds16, stream>>>(trans, sz, dA_array, ipiv_array,
-                                                                                dB_array);
-        MGI_LAUNCH_CHECK();
-        return MAGMA_SUCCESS;
-    }
+    if (max_nrhs > kRsCols && lds16 <= max_dynamic_lds())
+        return trans ? generic_launch<T, 2 * kRsCols, true>(sz, max_nrhs, lds16, dA_array, ipiv_array, dB_array, batch, stream)
+                     : generic_launch<T, 2 * kRsCols, false>(sz, max_nrhs, lds16, dA_array, ipiv_array, dB_array, batch, stream);
     const size_t lds = generic_bytes(sz.max_n, sizeof(T), kRsCols);
     if (lds > max_dynamic_lds()) return MAGMA_ERR_NOT_SUPPORTED;
-    MGI_HIP_CHECK(hipFuncSetAttribute((const void*)getrs_generic_kernel<T, kRsCols>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    dim3 grid(batch, ceildiv(max_nrhs, kRsCols));
-    getrs_generic_kernel<T, kRsCols><<<grid, kRsThreads, lds, stream>>>(trans, sz, dA_array, ipiv_array, dB_array);
-    MGI_LAUNCH_CHECK();
-    return MAGMA_SUCCESS;
+    return trans ? generic_launch<T, kRsCols, true>(sz, max_nrhs, lds, dA_array, ipiv_array, dB_array, batch, stream)
+                 : generic_launch<T, kRsCols, false>(sz, max_nrhs, lds, dA_array, ipiv_array, dB_array, batch, stream);
 }
 
 template magma_int_t getrs_generic_launch_t<double>(int, const RsSizes&, int, double**, int**, double**, int,
