@@ -468,10 +468,12 @@ getrs_wave_kernel(RsSizes sz, int groups, int nsys, T** __restrict__ dA_array, i
                 const int kk = kk0 - j;
                 if (kk >= 0) {
                     const int k = 64 * s + kk;
+                    // one reciprocal per step, independent of the chain
+                    const T rk = T(1) / dcur[j];
 #pragma unroll
                     for (int c = 0; c < kWaveCols; ++c) {
                         if (c < nr) {
-                            const T xk = readlane_t(x[s][c], kk) / dcur[j];
+                            const T xk = readlane_t(x[s][c], kk) * rk;
                             if (l == kk) x[s][c] = xk;
 #pragma unroll
                             for (int s2 = 0; s2 <= s; ++s2)
