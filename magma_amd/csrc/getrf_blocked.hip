@@ -702,7 +702,7 @@ static magma_int_t panel_launch(int R, const SizeArgs& sz, int k, T** dA_array, 
 // tile grid), y = system.
 template <typename T, int TM>
 __global__ void __launch_bounds__(256, 2)
-gemm_update_db_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
+gemm_update_db_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array, int mode)
 {
     constexpr int BK = 16, LDP = TM + 4;
     constexpr int WT = TM / 2;              // each wave's square of the tile
@@ -716,7 +716,13 @@ gemm_update_db_kernel(SizeArgs sz, int k, int nb, T** __restrict__ dA_array)
     const int minmn = min(m, n);
     if (k >= minmn) return;
     const int jb = min(nb, minmn - k);
-    const int M = m - k - jb, N = n - k - jb, K = jb;
+    int N = n - k - jb;
+    if (mode == kInPanelRight) {                    // only the block's second half
+        const int W = min(2 * nb, minmn - k);
+        if (W <= nb) return;
+        N = W - nb;
+    }
+    const int M = m - k - jb, K = jb;
     if (M <= 0 || N <= 0) return;
     const int tiles_m = (M + TM - 1) / TM;
     const int tid = blockIdx.x;
@@ -817,12 +823,12 @@ static magma_int_t trailing_update(const SizeArgs& sz, int k, int rest_m, int re
     if constexpr (MODE == kUpdF64 || MODE == kUpdF32) {
         if (rest_m >= 256 && rest_n >= 256) {
             const int tiles = ceildiv(rest_m, 128) * ceildiv(rest_n, 128);
-            gemm_update_db_kernel<T, 128><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
+            gemm_update_db_kernel<T, 128><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array, 0);
             MGI_LAUNCH_CHECK();
             return MAGMA_SUCCESS;
         }
         const int tiles = ceildiv(rest_m, 64) * ceildiv(rest_n, 64);
-        gemm_update_db_kernel<T, 64><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array);
+        gemm_update_db_kernel<T, 64><<<dim3(tiles, batch), 256, 0, stream>>>(sz, k, NB, dA_array, 0);
         MGI_LAUNCH_CHECK();
         return MAGMA_SUCCESS;
     }
@@ -883,8 +889,14 @@ static magma_int_t blocked_steps2(int R, const SizeArgs& sz, int max_m, int max_
         trsm_kernel<T, H><<<dim3(1, batch), 256, 0, stream>>>(sz, k, dA_array, kInPanelRight);
         MGI_LAUNCH_CHECK();
         if (max_m - k - H > 0) {
-            gemm_update_kernel<T, H, MODE><<<dim3(ceildiv(max_m - k - H, 64), 1, batch), 256, 0, stream>>>(
-                sz, k, H, dA_array, kInPanelRight);
+            if constexpr (MODE == kUpdF64 || MODE == kUpdF32) {
+                // at most H columns: one 64-wide column tile per row tile
+                gemm_update_db_kernel<T, 64><<<dim3(ceildiv(max_m - k - H, 64), batch), 256, 0, stream>>>(
+                    sz, k, H, dA_array, kInPanelRight);
+            } else {
+                gemm_update_kernel<T, H, MODE><<<dim3(ceildiv(max_m - k - H, 64), 1, batch), 256, 0, stream>>>(
+                    sz, k, H, dA_array, kInPanelRight);
+            }
             MGI_LAUNCH_CHECK();
         }
         // second half: panel, then its interchanges on the first half
